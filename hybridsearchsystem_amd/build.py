@@ -1,0 +1,74 @@
+"""Builds the in-tree shared library ``hybridsearchsystem_amd/libfrk.so`` for gfx950.
+
+Every ``csrc/*.hip`` file (device kernels and their C-ABI wrappers) and every ``csrc/**/*.cpp`` file
+(the C++ host engine and the handle API) is compiled with hipcc in parallel into ``_build/``, then
+linked. Run ``python -m hybridsearchsystem_amd.build`` (or ``__graft_entry__.build()``).
+"""
+import concurrent.futures as cf
+import glob
+import os
+import subprocess
+import sys
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+ROOT = os.path.dirname(HERE)
+CSRC = os.path.join(HERE, "csrc")
+BUILD = os.path.join(HERE, "_build")
+LIB = os.path.join(HERE, "libfrk.so")
+HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
+ARCH = "gfx950"
+
+COMMON = ["-O3", "-fPIC", "-std=c++17", "-Wall", "-Wno-unused-function", "-I" + os.path.join(ROOT, "include"),
+          "-I" + CSRC]
+# Packed FP32 ops are kept out of generated code (device-only flag; the host pass ignores it).
+DEVICE = ["--offload-arch=" + ARCH, "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
+          "-munsafe-fp-atomics"]
+
+
+def _sources():
+    hip = sorted(glob.glob(os.path.join(CSRC, "*.hip")))
+    cpp = sorted(glob.glob(os.path.join(CSRC, "**", "*.cpp"), recursive=True))
+    return hip, cpp
+
+
+def _obj(src):
+    rel = os.path.relpath(src, CSRC).replace(os.sep, "_")
+    return os.path.join(BUILD, rel + ".o")
+
+
+def _compile(src, is_hip):
+    obj = _obj(src)
+    cmd = [HIPCC] + COMMON + (["-x", "hip"] + DEVICE if is_hip else ["-D__HIP_PLATFORM_AMD__"]) + ["-c", src, "-o", obj]
+    p = subprocess.run(cmd, capture_output=True, text=True)
+    err = "\n".join(l for l in p.stderr.splitlines() if "packed-fp32-ops" not in l and l.strip())
+    return src, p.returncode, err
+
+
+def build(verbose=False, jobs=None):
+    os.makedirs(BUILD, exist_ok=True)
+    hip, cpp = _sources()
+    jobs = jobs or min(8, os.cpu_count() or 4)
+    failed = []
+    with cf.ThreadPoolExecutor(jobs) as ex:
+        futs = [ex.submit(_compile, s, True) for s in hip] + [ex.submit(_compile, s, False) for s in cpp]
+        for f in cf.as_completed(futs):
+            src, rc, err = f.result()
+            if rc != 0:
+                failed.append((src, err))
+            elif verbose and err:
+                print(err, file=sys.stderr)
+    if failed:
+        msg = "\n\n".join("%s:\n%s" % (s, e) for s, e in failed)
+        raise RuntimeError("hipcc failed:\n" + msg)
+    objs = [_obj(s) for s in hip + cpp]
+    tmp = LIB + ".tmp"
+    cmd = [HIPCC, "-shared", "-fPIC", "--offload-arch=" + ARCH, "-o", tmp] + objs + ["-lpthread"]
+    p = subprocess.run(cmd, capture_output=True, text=True)
+    if p.returncode != 0:
+        raise RuntimeError("link failed:\n" + p.stderr)
+    os.replace(tmp, LIB)
+    return LIB
+
+
+if __name__ == "__main__":
+    print(build(verbose=True))

@@ -1,0 +1,87 @@
+// frk_common.h -- shared definitions for the HIP kernels and their C-ABI wrappers (gfx950 only).
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <math.h>
+#include <stdint.h>
+
+#include "../../include/frk_capi.h"
+
+namespace frk {
+
+// Thread-local last-error text, returned by frk_last_error().
+void SetError(const char* fmt, ...);
+
+// Maps a failed HIP call to FRK_ERR_HIP with the call site in the error text.
+#define FRK_HIP_CHECK(expr)                                                              \
+    do {                                                                                 \
+        hipError_t e_ = (expr);                                                          \
+        if (e_ != hipSuccess) {                                                          \
+            ::frk::SetError("%s failed: %s (%s:%d)", #expr, hipGetErrorString(e_),        \
+                            __FILE__, __LINE__);                                         \
+            return FRK_ERR_HIP;                                                          \
+        }                                                                                \
+    } while (0)
+
+// Sortable candidate key: the high word orders by score (larger score -> larger key) and the low
+// word by doc id reversed (smaller id -> larger key), so the k largest keys are the top-k under
+// "score descending, then doc id ascending".
+__host__ __device__ inline uint32_t OrderedFromFloat(float f) {
+    if (f == 0.0f) f = 0.0f;  // -0.0 and +0.0 compare equal as scores
+    uint32_t u = __builtin_bit_cast(uint32_t, f);
+    return (u & 0x80000000u) ? ~u : (u | 0x80000000u);
+}
+
+__host__ __device__ inline float FloatFromOrdered(uint32_t k) {
+    uint32_t u = (k & 0x80000000u) ? (k & 0x7FFFFFFFu) : ~k;
+    return __builtin_bit_cast(float, u);
+}
+
+__host__ __device__ inline uint64_t MakeKey(float score, uint32_t doc) {
+    return (static_cast<uint64_t>(OrderedFromFloat(score)) << 32) | static_cast<uint64_t>(0xFFFFFFFFu - doc);
+}
+
+__host__ __device__ inline uint32_t KeyDoc(uint64_t key) { return 0xFFFFFFFFu - static_cast<uint32_t>(key); }
+
+__host__ __device__ inline float KeyScore(uint64_t key) { return FloatFromOrdered(static_cast<uint32_t>(key >> 32)); }
+
+// Round-to-nearest-even BF16 (NaN stays NaN).
+__host__ __device__ inline uint16_t F32ToBf16(float f) {
+    uint32_t u = __builtin_bit_cast(uint32_t, f);
+    if ((u & 0x7F800000u) == 0x7F800000u) return static_cast<uint16_t>((u >> 16) | ((u & 0xFFFFu) ? 0x40u : 0u));
+    u += 0x7FFFu + ((u >> 16) & 1u);  // round to nearest even
+    return static_cast<uint16_t>(u >> 16);
+}
+
+// OCP FP8 E4M3 (e4m3fn: bias 7, max 448, no infinities), round to nearest even, saturating.
+__host__ __device__ inline uint8_t F32ToE4M3(float f) {
+    uint32_t u = __builtin_bit_cast(uint32_t, f);
+    uint8_t sign = static_cast<uint8_t>((u >> 24) & 0x80u);
+    float a = fabsf(f);
+    if (!(a == a)) return static_cast<uint8_t>(sign | 0x7F);
+    if (a >= 448.0f) return static_cast<uint8_t>(sign | 0x7E);
+    if (a <= 0.0009765625f) return sign;  // at or below half the smallest subnormal (2^-10): ties to even 0
+    int e;
+    frexpf(a, &e);  // a = m * 2^e, m in [0.5, 1)
+    int exp_unb = e - 1;  // a = 1.x * 2^exp_unb
+    if (exp_unb < -6) {
+        // subnormal: value = k * 2^-9, k in 0..7
+        float k = a * 512.0f;
+        float r = rintf(k);  // nearest even
+        int ki = static_cast<int>(r);
+        if (ki >= 8) return static_cast<uint8_t>(sign | 0x08);  // rounds up to the smallest normal
+        return static_cast<uint8_t>(sign | ki);
+    }
+    float scaled = ldexpf(a, -exp_unb);  // [1, 2)
+    float mant = rintf((scaled - 1.0f) * 8.0f);  // 3 mantissa bits, nearest even
+    int m = static_cast<int>(mant);
+    if (m == 8) {
+        m = 0;
+        exp_unb += 1;
+    }
+    int be = exp_unb + 7;
+    if (be > 15 || (be == 15 && m > 6)) return static_cast<uint8_t>(sign | 0x7E);
+    return static_cast<uint8_t>(sign | (be << 3) | m);
+}
+
+}  // namespace frk

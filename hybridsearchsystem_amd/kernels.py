@@ -1,0 +1,81 @@
+"""Thin Python wrappers over the kernel-level C ABI (include/frk_capi.h, layer 1).
+
+Device buffers are torch tensors on the HIP device; every call is stream-ordered on torch's current
+stream unless a stream is given, and raises FrkError on a failed call.
+"""
+import ctypes
+
+import numpy as np
+
+from . import DocStore, FilterQuery, check, lib, ptr, stream_ptr
+
+FRK_DTYPE_F32, FRK_DTYPE_BF16, FRK_DTYPE_FP8E4M3 = 0, 1, 2
+
+
+def chunk_dims(dtype):
+    return int(lib().frk_chunk_dims(dtype))
+
+
+def padded_dim(dim, dtype):
+    cd = chunk_dims(dtype)
+    if dtype == FRK_DTYPE_FP8E4M3:
+        cd *= 2  # the FP8 MFMA consumes two chunks per step
+    return (dim + cd - 1) // cd * cd
+
+
+def tile_docs(vectors, dtype, dim_pad=None, threads=0):
+    """Row-major FP32 [n, dim] -> (tiles uint8 ndarray, scales float32 ndarray or None, dim_pad)."""
+    v = np.ascontiguousarray(vectors, dtype=np.float32)
+    n, dim = v.shape
+    dim_pad = dim_pad or padded_dim(dim, dtype)
+    nbytes = int(lib().frk_tiles_bytes(n, dim_pad, dtype))
+    if nbytes == 0:
+        raise ValueError("bad dim_pad %d for dtype %d" % (dim_pad, dtype))
+    tiles = np.empty(nbytes, dtype=np.uint8)
+    scales = np.empty(n, dtype=np.float32) if dtype == FRK_DTYPE_FP8E4M3 else None
+    check(lib().frk_tile_docs(v.ctypes.data, n, dim, dtype, dim_pad, tiles.ctypes.data,
+                              scales.ctypes.data if scales is not None else None, threads), "frk_tile_docs")
+    return tiles, scales, dim_pad
+
+
+def doc_store(tiles_dev, scales_dev, dtype, dim_pad, n_docs):
+    return DocStore(ptr(tiles_dev), ptr(scales_dev), dtype, dim_pad, n_docs)
+
+
+def filter_queries_array(descs):
+    arr = (FilterQuery * len(descs))()
+    for i, d in enumerate(descs):
+        arr[i] = FilterQuery(*d)
+    return np.frombuffer(bytes(arr), dtype=np.uint32).reshape(len(descs), 6)
+
+
+def filter_eval(d_queries, n_queries, d_expr, d_table, n_segments, seg_docs, n_docs, d_out, out_stride, d_counts,
+                d_status, stream=None):
+    check(lib().frk_filter_eval(ptr(d_queries), n_queries, ptr(d_expr), ptr(d_table), n_segments, seg_docs, n_docs,
+                                ptr(d_out), out_stride, ptr(d_counts), ptr(d_status), stream_ptr(stream)),
+          "frk_filter_eval")
+
+
+def score_select(store, d_queries, n_queries, d_masks, mask_stride, d_floor, block_begin, block_step, block_count,
+                 d_cand, cand_cap, d_counts, stream=None):
+    check(lib().frk_score_select(ctypes.byref(store), ptr(d_queries), n_queries, ptr(d_masks), mask_stride,
+                                 ptr(d_floor), block_begin, block_step, block_count, ptr(d_cand), cand_cap,
+                                 ptr(d_counts), stream_ptr(stream)), "frk_score_select")
+
+
+def select_topk(d_cand, cand_cap, d_counts, n_queries, d_k, k_default, d_out, k_stride, d_out_counts, d_kth=None,
+                stream=None):
+    check(lib().frk_select_topk(ptr(d_cand), cand_cap, ptr(d_counts), n_queries, ptr(d_k), k_default, ptr(d_out),
+                                k_stride, ptr(d_out_counts), ptr(d_kth), stream_ptr(stream)), "frk_select_topk")
+
+
+def decode_keys(keys):
+    """uint64 candidate keys -> (scores float32, doc ids int64)."""
+    keys = np.asarray(keys, dtype=np.uint64)
+    hi = (keys >> np.uint64(32)).astype(np.uint32)
+    lo = (keys & np.uint64(0xFFFFFFFF)).astype(np.uint32)
+    neg = (hi & np.uint32(0x80000000)) == 0
+    bits = np.where(neg, ~hi, hi & np.uint32(0x7FFFFFFF)).astype(np.uint32)
+    scores = bits.view(np.float32)
+    docs = (np.uint32(0xFFFFFFFF) - lo).astype(np.int64)
+    return scores, docs

@@ -1,0 +1,259 @@
+"""GPU parity of the kernel-level C ABI against the CPU oracle.
+
+- filter: bit-exact words for random ASTs over all five posting layouts (the oracle's per-doc
+  EvalFilterBitmap is the truth), plus match-all, absent tags and a ragged tail segment;
+- scorer + select: FP32 MFMA scores within 1e-5 of the FP64 dot product relative to sum|q_d v_d|,
+  top-k id sets equal to the FP32 CPU truth except for pairs inside that tolerance, exact ordering;
+- the probe floor (k-th best score of a sampled subset) leaves the top-k bit-identical.
+"""
+import numpy as np
+import pytest
+
+import oracle
+from tests import _gen
+
+pytestmark = pytest.mark.gpu
+
+torch = pytest.importorskip("torch")
+
+
+@pytest.fixture(scope="module")
+def frk():
+    if not torch.cuda.is_available():
+        pytest.skip("no GPU")
+    import hybridsearchsystem_amd as h
+    from hybridsearchsystem_amd import kernels
+    h.lib()
+    return kernels
+
+
+def dev(a, dtype=None):
+    t = torch.from_numpy(np.array(a, copy=True))
+    if dtype is not None:
+        t = t.to(dtype)
+    return t.cuda()
+
+
+def _filter_case(frk, seed, n_docs, seg_docs, n_tags, n_queries, depth, all_layouts=True):
+    print("filter case seed=%d n_docs=%d seg=%d tags=%d q=%d" % (seed, n_docs, seg_docs, n_tags, n_queries),
+          flush=True)
+    rng = np.random.default_rng(seed)
+    dens = np.exp(rng.uniform(np.log(0.0005), np.log(0.6), n_tags))
+    tags = _gen.tag_matrix(rng, n_docs, dens)
+    layout_of = lambda t: (t % 3 == 0, t % 2 == 0)  # noqa: E731 -- all five layouts appear
+    arena, offs, lays = _gen.build_posting_arena(tags, seg_docs, layout_of)
+    if all_layouts:
+        assert set(lays[lays >= 0].ravel().tolist()) == {0, 1, 2, 3, 4}
+    n_seg = offs.shape[1]
+    d_arena = dev(arena)
+    base = d_arena.data_ptr()
+    exprs, tables, descs, asts = [], [], [], []
+    expr_off = table_off = 0
+    for qi in range(n_queries):
+        if qi == 0:
+            ast = np.zeros((0, 2), np.int64)  # match all
+        elif qi == 1:
+            ast = oracle.parse_filter(str(int(rng.integers(0, n_tags))))  # a bare term
+        else:
+            ast = oracle.parse_filter(_gen.rand_filter(rng, n_tags, depth, absent_tags=3))
+        asts.append(ast)
+        expr, posts, ops = oracle.compile_postfix(ast)
+        flags = 1 if len(ast) == 0 else 0
+        tab = np.zeros((n_seg, len(posts)), dtype=np.uint64)
+        for p, tag in enumerate(posts):
+            for s in range(n_seg):
+                if tag < n_tags and offs[tag, s] >= 0:
+                    tab[s, p] = base + int(offs[tag, s])
+        descs.append((expr_off, len(expr), table_off, len(posts), ops, flags))
+        exprs.append(expr.astype(np.uint32))
+        tables.append(tab.ravel())
+        expr_off += len(expr)
+        table_off += tab.size
+    d_q = dev(frk.filter_queries_array(descs))
+    d_expr = dev(np.concatenate(exprs + [np.zeros(1, np.uint32)]).astype(np.int32))
+    d_tab = dev(np.concatenate(tables + [np.zeros(1, np.uint64)]).view(np.int64))
+    seg_words = seg_docs // 64
+    stride = n_seg * seg_words + 3
+    d_out = torch.full((n_queries, stride), -1, dtype=torch.int64, device="cuda")
+    d_cnt = torch.zeros(n_queries, dtype=torch.int32, device="cuda")
+    d_status = torch.zeros(1, dtype=torch.int32, device="cuda")
+    frk.filter_eval(d_q, n_queries, d_expr, d_tab, n_seg, seg_docs, n_docs, d_out, stride, d_cnt, d_status)
+    torch.cuda.synchronize()
+    assert int(d_status.item()) == 0
+    out = d_out.cpu().numpy().view(np.uint64)
+    cnt = d_cnt.cpu().numpy()
+    doc_words = _gen.doc_tag_words(tags)
+    for qi, ast in enumerate(asts):
+        want = oracle.filter_bitmap(ast, doc_words)
+        full = np.zeros(n_seg * seg_words, dtype=np.uint64)
+        full[:len(want)] = want
+        got = out[qi, :n_seg * seg_words]
+        assert np.array_equal(got, full), "query %d differs" % qi
+        assert out[qi, n_seg * seg_words:].tolist() == [np.uint64(2 ** 64 - 1)] * 3  # untouched past the stride
+        assert cnt[qi] == int(sum(bin(int(w)).count("1") for w in want))
+
+
+def test_filter_bit_exact_all_layouts(frk):
+    _filter_case(frk, seed=1, n_docs=3 * 8192 + 1000, seg_docs=8192, n_tags=40, n_queries=60, depth=4)
+
+
+def test_filter_bit_exact_full_segments(frk):
+    # reference segment size (131072 docs = 2048 words, 8 chunks per segment), short tail segment
+    _filter_case(frk, seed=2, n_docs=131072 + 19264, seg_docs=131072, n_tags=24, n_queries=20, depth=3)
+
+
+def test_filter_single_segment_tiny(frk):
+    _filter_case(frk, seed=3, n_docs=100, seg_docs=128, n_tags=6, n_queries=12, depth=2, all_layouts=False)
+
+
+def test_filter_many_sparse_operands(frk):
+    # more sparse operands than LDS slots: the per-thread fallback path
+    rng = np.random.default_rng(4)
+    n_docs, seg_docs, n_tags = 2 * 4096 + 77, 4096, 60
+    tags = _gen.tag_matrix(rng, n_docs, np.full(n_tags, 0.003))
+    arena, offs, lays = _gen.build_posting_arena(tags, seg_docs, lambda t: (t % 4 == 0, t % 2 == 1))
+    n_seg = offs.shape[1]
+    d_arena = dev(arena)
+    expr_s = " OR ".join(str(t) for t in range(50))
+    ast = oracle.parse_filter("(" + expr_s + ") AND NOT (55 OR 56)")
+    expr, posts, ops = oracle.compile_postfix(ast)
+    tab = np.zeros((n_seg, len(posts)), dtype=np.uint64)
+    for p, tag in enumerate(posts):
+        for s in range(n_seg):
+            if offs[tag, s] >= 0:
+                tab[s, p] = d_arena.data_ptr() + int(offs[tag, s])
+    d_q = dev(frk.filter_queries_array([(0, len(expr), 0, len(posts), ops, 0)]))
+    stride = n_seg * seg_docs // 64
+    d_out = torch.zeros((1, stride), dtype=torch.int64, device="cuda")
+    d_cnt = torch.zeros(1, dtype=torch.int32, device="cuda")
+    d_status = torch.zeros(1, dtype=torch.int32, device="cuda")
+    frk.filter_eval(d_q, 1, dev(expr.astype(np.int32)), dev(tab.ravel().view(np.int64)), n_seg, seg_docs, n_docs,
+                    d_out, stride, d_cnt, d_status)
+    torch.cuda.synchronize()
+    want = oracle.filter_bitmap(ast, _gen.doc_tag_words(tags))
+    got = d_out.cpu().numpy().view(np.uint64)[0, :len(want)]
+    assert int(d_status.item()) == 0
+    assert np.array_equal(got, want)
+
+
+def test_filter_rejects_malformed_program(frk):
+    # an AND that asks for a stack entry that does not exist: flagged, no out-of-bounds access
+    d_q = dev(frk.filter_queries_array([(0, 3, 0, 0, 1, 0)]))
+    d_out = torch.zeros((1, 2), dtype=torch.int64, device="cuda")
+    d_cnt = torch.zeros(1, dtype=torch.int32, device="cuda")
+    d_status = torch.zeros(1, dtype=torch.int32, device="cuda")
+    frk.filter_eval(d_q, 1, dev(np.array([1, 0, 1], np.int32)), dev(np.zeros(1, np.int64)), 1, 128, 100, d_out, 2,
+                    d_cnt, d_status)
+    torch.cuda.synchronize()
+    assert int(d_status.item()) != 0
+    assert int(d_out.abs().sum().item()) == 0
+
+
+# ------------------------------------------------------------------------------------- scorer + select
+def _run_topk(frk, docs, qs, masks_words, ks, dtype=0, floor=None, block_step=1, cap=None):
+    n, dim = docs.shape
+    tiles, scales, dim_pad = frk.tile_docs(docs, dtype)
+    store_t = dev(tiles)
+    scales_t = dev(scales) if scales is not None else None
+    store = frk.doc_store(store_t, scales_t, dtype, dim_pad, n)
+    qp = np.zeros((qs.shape[0], dim_pad), np.float32)
+    qp[:, :dim] = qs
+    nq = qs.shape[0]
+    d_q = dev(qp)
+    mw = (n + 63) // 64
+    d_m = dev(masks_words.view(np.int64)) if masks_words is not None else None
+    cap = cap or n
+    d_cand = torch.zeros((nq, cap), dtype=torch.int64, device="cuda")
+    d_cnt = torch.zeros(nq, dtype=torch.int32, device="cuda")
+    n_blocks = (n + 31) // 32
+    count = (n_blocks + block_step - 1) // block_step
+    d_floor = dev(floor.astype(np.float32)) if floor is not None else None
+    frk.score_select(store, d_q, nq, d_m, mw, d_floor, 0, block_step, count, d_cand, cap, d_cnt)
+    kmax = int(max(ks))
+    d_k = dev(np.asarray(ks, np.int32))
+    d_out = torch.zeros((nq, kmax), dtype=torch.int64, device="cuda")
+    d_oc = torch.zeros(nq, dtype=torch.int32, device="cuda")
+    d_kth = torch.zeros(nq, dtype=torch.float32, device="cuda")
+    frk.select_topk(d_cand, cap, d_cnt, nq, d_k, 0, d_out, kmax, d_oc, d_kth)
+    torch.cuda.synchronize()
+    return (d_out.cpu().numpy().view(np.uint64), d_oc.cpu().numpy(), d_cnt.cpu().numpy(), d_kth.cpu().numpy())
+
+
+def _check_against_truth(frk, docs, qs, bits, ks, keys, oc, rel_tol=1e-5):
+    masks = np.stack([oracle.words_from_bits(b) for b in bits])
+    ids32, sc32, cnt32 = oracle.topk(docs, qs, ks, masks, mode=0)
+    d64 = docs.astype(np.float64)
+    for q in range(qs.shape[0]):
+        k = int(ks[q])
+        n_pass = int(bits[q].sum())
+        assert oc[q] == min(k, n_pass)
+        scores, ids = frk.decode_keys(keys[q, :oc[q]])
+        # ordering: score descending, then doc id ascending
+        for a in range(len(ids) - 1):
+            assert scores[a] > scores[a + 1] or (scores[a] == scores[a + 1] and ids[a] < ids[a + 1])
+        assert all(bits[q][ids])
+        exact = d64[ids] @ qs[q].astype(np.float64)
+        scale = np.abs(d64[ids] * qs[q].astype(np.float64)).sum(axis=1)
+        assert np.all(np.abs(scores - exact) <= rel_tol * scale + 1e-30)
+        truth = set(ids32[q, :cnt32[q]].tolist())
+        got = set(ids.tolist())
+        if got != truth:
+            # only near-ties at the k-th boundary may differ
+            allowed = rel_tol * np.abs(d64 * qs[q].astype(np.float64)).sum(axis=1).max()
+            kth = sc32[q, cnt32[q] - 1]
+            for d in got ^ truth:
+                assert abs(float(d64[d] @ qs[q].astype(np.float64)) - kth) <= 2 * allowed, (q, d)
+
+
+@pytest.mark.parametrize("n,dim,nq", [(5000, 128, 64), (4096 + 17, 256, 100), (1000, 64, 7), (20000, 256, 128)])
+def test_fp32_topk_matches_cpu_truth(frk, n, dim, nq):
+    print("fp32 topk n=%d dim=%d nq=%d" % (n, dim, nq), flush=True)
+    rng = np.random.default_rng(n + dim + nq)
+    docs, cent = _gen.mixture_corpus(rng, n, dim, 16)
+    qs = _gen.mixture_queries(rng, cent, nq)
+    bits = rng.random((nq, n)) < rng.uniform(0.001, 0.7, (nq, 1))
+    ks = rng.choice([1, 10, 100, 1000], size=nq).astype(np.int32)
+    masks = np.stack([oracle.words_from_bits(b) for b in bits])
+    keys, oc, cnt, kth = _run_topk(frk, docs, qs, masks, ks)
+    assert np.array_equal(cnt, bits.sum(axis=1))  # unfloored: every passing doc is a candidate
+    _check_against_truth(frk, docs, qs, bits, ks, keys, oc)
+
+
+def test_fp32_ties_broken_by_doc_id(frk):
+    rng = np.random.default_rng(21)
+    n, dim, nq = 3000, 64, 8
+    base = rng.standard_normal((40, dim)).astype(np.float32)
+    docs = base[rng.integers(0, 40, n)]  # many exact duplicates -> exact score ties
+    qs = rng.standard_normal((nq, dim)).astype(np.float32)
+    bits = rng.random((nq, n)) < 0.5
+    ks = np.full(nq, 50, np.int32)
+    keys, oc, cnt, kth = _run_topk(frk, docs, qs, np.stack([oracle.words_from_bits(b) for b in bits]), ks)
+    ids64, _, c64 = oracle.topk(docs, qs, ks, np.stack([oracle.words_from_bits(b) for b in bits]), mode=1)
+    for q in range(nq):
+        _, ids = frk.decode_keys(keys[q, :oc[q]])
+        assert ids.tolist() == ids64[q, :c64[q]].tolist()
+
+
+def test_probe_floor_keeps_topk_identical(frk):
+    rng = np.random.default_rng(33)
+    n, dim, nq = 40000, 128, 64
+    docs, cent = _gen.mixture_corpus(rng, n, dim, 32)
+    qs = _gen.mixture_queries(rng, cent, nq)
+    bits = rng.random((nq, n)) < rng.uniform(0.01, 0.6, (nq, 1))
+    masks = np.stack([oracle.words_from_bits(b) for b in bits])
+    ks = rng.choice([10, 100], size=nq).astype(np.int32)
+    keys0, oc0, cnt0, _ = _run_topk(frk, docs, qs, masks, ks)
+    _, _, _, kth = _run_topk(frk, docs, qs, masks, ks, block_step=8)  # probe: every 8th block
+    keys1, oc1, cnt1, _ = _run_topk(frk, docs, qs, masks, ks, floor=kth)
+    assert np.array_equal(oc0, oc1)
+    for q in range(nq):
+        assert keys0[q, :oc0[q]].tolist() == keys1[q, :oc1[q]].tolist()
+    assert cnt1.sum() < cnt0.sum() / 3  # the floor prunes most candidates
+
+
+def test_candidate_overflow_is_reported(frk):
+    rng = np.random.default_rng(5)
+    docs = rng.standard_normal((3000, 64)).astype(np.float32)
+    qs = rng.standard_normal((4, 64)).astype(np.float32)
+    keys, oc, cnt, kth = _run_topk(frk, docs, qs, None, np.full(4, 10, np.int32), cap=100)
+    assert np.all(cnt == 3000)  # counts keep counting past the capacity; the caller detects cnt > cap
