@@ -1,0 +1,205 @@
+"""Benchmark: filtered top-100 search over 10M x 256 docs on one MI355X.
+
+One step = one batch of 128 queries through NpuRetrieval::FullRecallSearcher (filter over the posting
+lists, fused MFMA score + mask + floor + compaction, top-k select, D2H). The chosen pipeline (BF16
+docs on MFMA, pruning floor, FP32 rescoring of the top k*m) is timed beside the plain FP32 pipeline on
+the same resident index and query stream; `value` is the chosen pipeline's queries/s, reported only
+if its recall@100 against the plain FP32 results is >= 0.99 (else the plain path's QPS).
+
+Prints one JSON line. Inputs are synthetic (seeded Gaussian-mixture corpus, Zipf tag densities).
+"""
+import argparse
+import json
+import os
+import sys
+import threading
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+METRIC = "QPS@recall≥0.99 + p99 ms, 10M×256 filtered top-100; chosen vs plain FP32"
+HBM_PEAK_GBS = 8000.0
+MFMA_PEAK_TFLOPS = {"f32": 157.3, "bf16": 2500.0, "fp8": 5000.0}
+
+
+def log(*a):
+    print(*a, file=sys.stderr, flush=True)
+
+
+def build_table(n_docs, dim, n_tags, centroids, dtypes, seed, keep_rows, chunk=1 << 20):
+    import torch
+    from hybridsearchsystem_amd import search, synth
+    t0 = time.time()
+    dens = synth.zipf_densities(n_tags)
+    off, ids = synth.gen_postings(np.random.default_rng(synth.SEEDS["tags"]), n_docs, dens)
+    log("postings: %d ids over %d tags in %.1fs" % (len(ids), n_tags, time.time() - t0))
+    vec = torch.empty((n_docs, dim), dtype=torch.float32, device="cuda")
+    for at in range(0, n_docs, chunk):
+        m = min(chunk, n_docs - at)
+        vec[at:at + m] = synth.mixture_vectors_torch(seed, m, dim, centroids, first=at)
+    torch.cuda.synchronize()
+    log("vectors generated on the GPU in %.1fs" % (time.time() - t0))
+    table = search.Table.build(vec, off, ids, n_docs=n_docs, dtypes=dtypes, docs_per_segment=131072)
+    head = vec[:keep_rows].cpu().numpy()  # the CPU baseline scans the same docs
+    del vec
+    torch.cuda.empty_cache()
+    log("table built in %.1fs" % (time.time() - t0))
+    return table, dens, off, ids, head
+
+
+def run_steps(table, queries, filters, ks, steps, batch, **kw):
+    """Times `steps` batches; returns (per-batch ms list, last stats, results of every batch)."""
+    n = queries.shape[0]
+    lat, results, stats = [], [], []
+    for s in range(steps):
+        lo = (s * batch) % n
+        sl = slice(lo, lo + batch)
+        t0 = time.perf_counter()
+        r = table.search(queries[sl], filters[sl], ks[sl], **kw)
+        lat.append((time.perf_counter() - t0) * 1e3)
+        results.append((lo, r[0], r[2]))
+        stats.append(r[4])
+    return lat, stats, results
+
+
+def recall(chosen, plain):
+    hits = tot = 0
+    for (lo_c, ids_c, cnt_c), (lo_p, ids_p, cnt_p) in zip(chosen, plain):
+        assert lo_c == lo_p
+        for q in range(ids_p.shape[0]):
+            truth = set(ids_p[q, :cnt_p[q]].tolist())
+            hits += len(truth & set(ids_c[q, :cnt_c[q]].tolist()))
+            tot += len(truth)
+    return hits / max(tot, 1)
+
+
+def cpu_baseline(docs, dens, off, ids, queries, filters, ks, n_docs_full, threads):
+    """The reference's CPU brute force (CpuTopKBatch, 10.13 recall_ref.cpp:90-140, compiled from the
+    reference sources into oracle/_ref) over a doc sample, threads over disjoint doc ranges; else
+    the oracle's port of it. Scaled to queries/s over the full corpus."""
+    import oracle
+    from hybridsearchsystem_amd import synth
+    sample_docs = docs.shape[0]
+    words = synth.doc_tag_words(off, ids, n_docs_full, len(dens), 0, sample_docs)
+    asts = [oracle.parse_filter(f) for f in filters]
+    use_ref = oracle.ref() is not None
+    per = (sample_docs + threads - 1) // threads
+    out = [None] * threads
+
+    def work(t):
+        lo, hi = t * per, min(sample_docs, (t + 1) * per)
+        if lo >= hi:
+            return
+        if use_ref:
+            out[t] = oracle.ref_cpu_topk_batch(docs, words, queries, asts, ks, fp32=True, doc_offset=lo,
+                                               n_scan=hi - lo)
+        else:
+            masks = np.stack([oracle.filter_bitmap(a, words[lo:hi]) for a in asts])
+            out[t] = oracle.topk(docs[lo:hi], queries, ks, masks, mode=0, threads=1)
+
+    t0 = time.perf_counter()
+    th = [threading.Thread(target=work, args=(t,)) for t in range(threads)]
+    for x in th:
+        x.start()
+    for x in th:
+        x.join()
+    secs = time.perf_counter() - t0
+    qps = queries.shape[0] / (secs * n_docs_full / sample_docs)
+    return dict(value=round(qps, 3), unit="queries/s", cores=threads, kind="reference" if use_ref else "port",
+                sample="%d queries x first %d of %d docs (%.1fs), scaled by docs" % (queries.shape[0], sample_docs,
+                                                                                      n_docs_full, secs))
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=20)
+    ap.add_argument("--warmup", type=int, default=3)
+    ap.add_argument("--docs", type=int, default=10_000_000)
+    ap.add_argument("--dim", type=int, default=256)
+    ap.add_argument("--batch", type=int, default=128)
+    ap.add_argument("--k", type=int, default=100)
+    ap.add_argument("--queries", type=int, default=1024)
+    ap.add_argument("--cpu-sample", type=int, default=4_000_000)
+    ap.add_argument("--cpu-threads", type=int, default=16)
+    ap.add_argument("--no-cpu", action="store_true")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print("bench.py runs on one GPU: this build has no multi-GPU path (--gpus %d)" % args.gpus, file=sys.stderr)
+        sys.exit(2)
+    import torch
+    from hybridsearchsystem_amd import synth
+    from hybridsearchsystem_amd.search import FRK_DTYPE_BF16, FRK_DTYPE_F32
+
+    n_tags, centroids, seed = 1024, 1024, synth.SEEDS["corpus"]
+    table, dens, off, ids, head = build_table(args.docs, args.dim, n_tags, centroids, (FRK_DTYPE_BF16,), seed,
+                                              0 if args.no_cpu else min(args.cpu_sample, args.docs))
+    queries = synth.mixture_queries(synth.SEEDS["queries"], args.queries, args.dim, centroids, corpus_seed=seed)
+    filters, sels = synth.random_filters(synth.SEEDS["filters"], dens, args.queries, depth=3, sel=(0.0001, 0.6))
+    ks = np.full(args.queries, args.k, np.uint32)
+
+    plain_kw = dict(dtype=FRK_DTYPE_F32)
+    chosen_kw = dict(dtype=FRK_DTYPE_BF16, rescore_factor=4)
+    run_steps(table, queries, filters, ks, args.warmup, args.batch, **plain_kw)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    lat_p, st_p, res_p = run_steps(table, queries, filters, ks, args.steps, args.batch, **plain_kw)
+    torch.cuda.synchronize()
+    wall_p = time.perf_counter() - t0
+    run_steps(table, queries, filters, ks, args.warmup, args.batch, **chosen_kw)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    lat_c, st_c, res_c = run_steps(table, queries, filters, ks, args.steps, args.batch, **chosen_kw)
+    torch.cuda.synchronize()
+    wall_c = time.perf_counter() - t0
+    rec = recall(res_c, res_p)
+    qps_p = args.steps * args.batch / wall_p
+    qps_c = args.steps * args.batch / wall_c
+    chosen_ok = rec >= 0.99
+    value = qps_c if chosen_ok else qps_p
+    lat = lat_c if chosen_ok else lat_p
+    st = st_c if chosen_ok else st_p
+
+    # roofline of the dominant kernel: the main fused score pass (HIP events around its launch)
+    n, d, q = args.docs, args.dim, args.batch
+    doc_bytes = ((n + 31) // 32) * 32 * d * (2 if chosen_ok else 4)
+    mask_bytes = q * ((n + 63) // 64) * 8
+    alg_bytes = doc_bytes + mask_bytes
+    ms_score = float(np.mean([s["ms_score"] for s in st]))
+    achieved = alg_bytes / (ms_score * 1e-3) / 1e9
+    roof = dict(bound="hbm", achieved=round(achieved, 1), peak=HBM_PEAK_GBS, unit="GB/s",
+                frac=round(achieved / HBM_PEAK_GBS, 4), traffic=None)
+    ms_score_p = float(np.mean([s["ms_score"] for s in st_p]))
+    flops_p = 2.0 * q * n * d
+    out = {
+        "metric": METRIC, "value": round(value, 2), "unit": "queries/s", "n_gpus": 1, "steps": args.steps,
+        "warmup": args.warmup, "ms_per_step": round(wall_c / args.steps * 1e3 if chosen_ok else wall_p / args.steps * 1e3, 3),
+        "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "bf16" if chosen_ok else "f32",
+        "data": "synthetic (seeded Gaussian mixture, Zipf tags)",
+        "config": {"workload": "10M x 256 filtered top-%d, batch %d" % (args.k, args.batch), "docs": n, "dim": d,
+                   "batch": q, "k": args.k, "tags": n_tags, "selectivity": [0.0001, 0.6]},
+        "p50_ms": round(float(np.percentile(lat, 50)), 3), "p99_ms": round(float(np.percentile(lat, 99)), 3),
+        "recall_at_k_vs_plain": round(rec, 5),
+        "plain_fp32": {"qps": round(qps_p, 2), "p50_ms": round(float(np.percentile(lat_p, 50)), 3),
+                       "p99_ms": round(float(np.percentile(lat_p, 99)), 3), "score_kernel_ms": round(ms_score_p, 3),
+                       "score_tflops": round(flops_p / (ms_score_p * 1e-3) / 1e12, 1)},
+        "chosen": {"qps": round(qps_c, 2), "pipeline": "bf16 + probe floor + fp32 rescore of top k*4",
+                   "speedup_vs_plain": round(qps_c / qps_p, 3)},
+        "stages_ms": {k: round(float(np.mean([s[k] for s in st])), 3) for k in
+                      ("ms_filter", "ms_probe", "ms_score", "ms_select", "ms_rescore", "ms_total")},
+        "candidates_per_query": round(float(np.mean([s["candidates"] for s in st])) / q, 1),
+        "plain_candidates_per_query": round(float(np.mean([s["candidates"] for s in st_p])) / q, 1),
+        "probe_step": int(st[-1]["probe_step"]), "plain_probe_step": int(st_p[-1]["probe_step"]),
+        "roofline": roof,
+    }
+    if not args.no_cpu:
+        out["cpu_baseline"] = cpu_baseline(head, dens, off, ids, queries[:args.batch], filters[:args.batch],
+                                           ks[:args.batch], n, args.cpu_threads)
+    print(json.dumps(out), flush=True)
+
+
+if __name__ == "__main__":
+    main()

@@ -9,7 +9,7 @@ import ctypes
 import os
 
 HERE = os.path.dirname(os.path.abspath(__file__))
-LIB_PATH = os.path.join(HERE, "libfrk.so")
+LIB_PATH = os.environ.get("FRK_LIB_PATH") or os.path.join(HERE, "libfrk.so")
 
 FRK_DTYPE_F32, FRK_DTYPE_BF16, FRK_DTYPE_FP8E4M3 = 0, 1, 2
 FILTER_MATCH_ALL = 1
@@ -47,7 +47,7 @@ def lib():
     L.frk_chunk_dims.restype = u32
     L.frk_chunk_dims.argtypes = [u32]
     L.frk_score_select.restype = i32
-    L.frk_score_select.argtypes = [ctypes.POINTER(DocStore), vp, u32, vp, u64, vp, u64, u64, u64, vp, u32, vp, vp]
+    L.frk_score_select.argtypes = [ctypes.POINTER(DocStore), vp, u32, vp, u32, vp, u64, u64, u64, vp, u32, vp, vp]
     L.frk_select_topk.restype = i32
     L.frk_select_topk.argtypes = [vp, u32, vp, u32, vp, u32, vp, u32, vp, vp, vp]
     _bind_optional(L)

@@ -38,7 +38,9 @@ def _obj(src):
 
 def _compile(src, is_hip):
     obj = _obj(src)
-    cmd = [HIPCC] + COMMON + (["-x", "hip"] + DEVICE if is_hip else ["-D__HIP_PLATFORM_AMD__"]) + ["-c", src, "-o", obj]
+    # host-only C++ still goes through hipcc (HIP runtime headers); the arch flag keeps it from
+    # compiling for a default device target
+    cmd = [HIPCC] + COMMON + (["-x", "hip"] + DEVICE if is_hip else ["--offload-arch=" + ARCH]) + ["-c", src, "-o", obj]
     p = subprocess.run(cmd, capture_output=True, text=True)
     err = "\n".join(l for l in p.stderr.splitlines() if "packed-fp32-ops" not in l and l.strip())
     return src, p.returncode, err

@@ -1,0 +1,241 @@
+// capi_engine.cpp -- the handle layer of the C ABI (include/frk_capi.h, layer 2): DataTable loading
+// and building, and batch search with filter strings, for callers that cannot link C++.
+#include <hip/hip_runtime.h>
+
+#include <cstring>
+#include <memory>
+#include <string>
+#include <vector>
+
+#include "../../../include/frk_capi.h"
+#include "data_table.h"
+#include "full_recall_searcher.h"
+#include "log.h"
+#include "query_node.h"
+
+using namespace NpuRetrieval;
+
+struct frk_table {
+    std::shared_ptr<DataTable> table;
+};
+
+struct frk_builder {
+    std::unique_ptr<TableBuilder> b;
+};
+
+namespace {
+int Fail(int code, const char* what) {
+    frk::SetError("%s", what);
+    return code;
+}
+
+void FilterToPairs(const FilterAst* n, std::vector<int64_t>& out) {
+    switch (n->op) {
+        case FilterAst::Term:
+            out.push_back(0);
+            out.push_back(static_cast<int64_t>(n->tag));
+            return;
+        case FilterAst::And:
+        case FilterAst::Or:
+            out.push_back(n->op == FilterAst::And ? 1 : 2);
+            out.push_back(static_cast<int64_t>(n->children.size()));
+            break;
+        case FilterAst::Not:
+            out.push_back(3);
+            out.push_back(1);
+            break;
+    }
+    for (const auto& c : n->children) FilterToPairs(c.get(), out);
+}
+}  // namespace
+
+extern "C" {
+
+int frk_table_load(int32_t device, const char* dir, uint32_t dtype_mask, frk_table** out) {
+    if (!dir || !out) return Fail(FRK_ERR_INVALID, "frk_table_load: null argument");
+    auto t = std::make_shared<DataTable>();
+    if (!t->LoadData(device, dir, dtype_mask)) return FRK_ERR_IO;
+    *out = new frk_table{t};
+    return FRK_OK;
+}
+
+void frk_table_free(frk_table* t) { delete t; }
+
+int frk_table_info(const frk_table* t, uint64_t* n_docs, uint32_t* n_segments, uint32_t* docs_per_segment) {
+    if (!t) return Fail(FRK_ERR_INVALID, "frk_table_info: null table");
+    if (n_docs) *n_docs = t->table->GetDocNum();
+    if (n_segments) *n_segments = t->table->GetSegmentNum();
+    if (docs_per_segment) *docs_per_segment = t->table->GetDocNumPerSegment();
+    return FRK_OK;
+}
+
+int frk_builder_create(int32_t device, uint32_t n_docs, uint32_t docs_per_segment, frk_builder** out) {
+    if (!out || n_docs == 0) return Fail(FRK_ERR_INVALID, "frk_builder_create: bad arguments");
+    auto* b = new frk_builder{std::make_unique<TableBuilder>(device, n_docs, docs_per_segment)};
+    if (!b->b->Error().empty()) {
+        std::string e = b->b->Error();
+        delete b;
+        return Fail(FRK_ERR_INVALID, e.c_str());
+    }
+    *out = b;
+    return FRK_OK;
+}
+
+void frk_builder_free(frk_builder* b) { delete b; }
+
+int frk_builder_add_vector_field(frk_builder* b, const char* name, uint32_t dim, uint32_t dtype_mask) {
+    if (!b || !name || dim == 0) return Fail(FRK_ERR_INVALID, "frk_builder_add_vector_field: bad arguments");
+    return b->b->AddVectorField(name, dim, dtype_mask) ? FRK_OK : Fail(FRK_ERR_NOMEM, b->b->Error().c_str());
+}
+
+int frk_builder_set_vectors(frk_builder* b, const char* name, const float* rows, uint64_t first, uint64_t n,
+                            int rows_on_device) {
+    if (!b || !name || !rows) return Fail(FRK_ERR_INVALID, "frk_builder_set_vectors: bad arguments");
+    return b->b->SetVectors(name, rows, first, n, rows_on_device != 0) ? FRK_OK
+                                                                       : Fail(FRK_ERR_INVALID, b->b->Error().c_str());
+}
+
+int frk_builder_add_posting_field(frk_builder* b, const char* name, int is_matrix, float density_threshold,
+                                  int packed) {
+    if (!b || !name) return Fail(FRK_ERR_INVALID, "frk_builder_add_posting_field: bad arguments");
+    return b->b->AddPostingField(name, is_matrix != 0, density_threshold, packed != 0) ? FRK_OK : FRK_ERR_INVALID;
+}
+
+int frk_builder_add_tag_postings(frk_builder* b, const char* field, uint64_t n_tags, const uint64_t* tags,
+                                 const uint64_t* offsets, const uint32_t* doc_ids) {
+    if (!b || !field || !tags || !offsets || (!doc_ids && offsets[n_tags] != 0))
+        return Fail(FRK_ERR_INVALID, "frk_builder_add_tag_postings: bad arguments");
+    for (uint64_t t = 0; t < n_tags; t++) {
+        const uint64_t lo = offsets[t], hi = offsets[t + 1];
+        if (hi < lo) return Fail(FRK_ERR_INVALID, "frk_builder_add_tag_postings: offsets must not decrease");
+        for (uint64_t i = lo + 1; i < hi; i++)
+            if (doc_ids[i] <= doc_ids[i - 1]) return Fail(FRK_ERR_INVALID, "posting doc ids must be strictly ascending");
+        if (hi == lo) continue;
+        if (!b->b->AddTokenPostings(field, TagToken(tags[t]), doc_ids + lo, hi - lo))
+            return Fail(FRK_ERR_INVALID, b->b->Error().c_str());
+    }
+    return FRK_OK;
+}
+
+int frk_builder_set_gdocids(frk_builder* b, const uint64_t* ids, uint64_t n) {
+    if (!b || !ids) return Fail(FRK_ERR_INVALID, "frk_builder_set_gdocids: bad arguments");
+    b->b->SetGDocIds(ids, n);
+    return FRK_OK;
+}
+
+int frk_builder_finish(frk_builder* b, frk_table** out) {
+    if (!b || !out) return Fail(FRK_ERR_INVALID, "frk_builder_finish: null argument");
+    auto t = b->b->Finish();
+    if (!t) return Fail(FRK_ERR_HIP, b->b->Error().c_str());
+    *out = new frk_table{t};
+    return FRK_OK;
+}
+
+uint64_t frk_hash64(const char* data, uint32_t len) { return Hash64(data, len); }
+
+int64_t frk_serialize_posting(const uint32_t* ids, uint32_t n_ids, uint32_t doc_num, int is_matrix, float threshold,
+                              int packed, uint8_t* out, uint64_t cap) {
+    std::vector<uint32_t> v(ids, ids + n_ids);
+    std::string buf;
+    if (!SerializePosting(v, doc_num, is_matrix != 0, threshold, packed != 0, buf))
+        return Fail(FRK_ERR_INVALID, "posting does not serialize");
+    if (buf.size() > cap) return Fail(FRK_ERR_CAPACITY, "frk_serialize_posting: output buffer too small");
+    std::memcpy(out, buf.data(), buf.size());
+    return static_cast<int64_t>(buf.size());
+}
+
+int frk_parse_filter(const char* expr, int64_t* pairs, int32_t cap_pairs) {
+    if (!expr) return Fail(FRK_ERR_INVALID, "frk_parse_filter: null expression");
+    bool ok = true;
+    auto ast = ParseFilter(expr, ok);
+    if (!ok) return Fail(FRK_ERR_INVALID, "filter does not parse");
+    if (!ast) return 0;
+    std::vector<int64_t> p;
+    FilterToPairs(ast.get(), p);
+    const int32_t n = static_cast<int32_t>(p.size() / 2);
+    if (n > cap_pairs) return Fail(FRK_ERR_CAPACITY, "frk_parse_filter: pair buffer too small");
+    std::memcpy(pairs, p.data(), p.size() * sizeof(int64_t));
+    return n;
+}
+
+int frk_compile_filter(const frk_table* t, const char* field, const char* expr, uint32_t* out_expr, int32_t cap,
+                       uint32_t* n_postings, uint32_t* op_num, int32_t* match_all) {
+    if (!t || !field || !expr || !out_expr || !n_postings || !op_num || !match_all)
+        return Fail(FRK_ERR_INVALID, "frk_compile_filter: null argument");
+    bool ok = true;
+    auto ast = ParseFilter(expr, ok);
+    if (!ok) return Fail(FRK_ERR_INVALID, "filter does not parse");
+    auto tree = BuildQueryNode(ast.get(), field);
+    FilterExpr fe;
+    if (!PrepareFilterExpr(*t->table, tree.get(), fe)) return FRK_ERR_INVALID;
+    if (static_cast<int32_t>(fe.postExpr.size()) > cap) return Fail(FRK_ERR_CAPACITY, "program buffer too small");
+    std::memcpy(out_expr, fe.postExpr.data(), fe.postExpr.size() * 4);
+    *n_postings = static_cast<uint32_t>(fe.postingDeviceAddrs.size());
+    *op_num = fe.opNum;
+    *match_all = fe.emptyTree ? 1 : 0;
+    return static_cast<int>(fe.postExpr.size());
+}
+
+int frk_search(const frk_table* t, const char* vector_field, const char* posting_field, uint32_t n_queries,
+               const float* queries, uint32_t dim, const char* const* filters, const uint32_t* ks,
+               const frk_search_options* opts, uint32_t k_stride, int64_t* out_ids, float* out_scores,
+               uint32_t* out_counts, uint32_t* out_passing, frk_search_stats* stats) {
+    if (!t || !vector_field || !posting_field || (!queries && n_queries) || !ks || !out_ids || !out_counts)
+        return Fail(FRK_ERR_INVALID, "frk_search: null argument");
+    std::vector<std::unique_ptr<QueryNode>> trees(n_queries);
+    FullRecallSearcher searcher(t->table->GetDeviceId(), t->table);
+    for (uint32_t q = 0; q < n_queries; q++) {
+        if (ks[q] > k_stride) return Fail(FRK_ERR_CAPACITY, "frk_search: k larger than k_stride");
+        bool ok = true;
+        std::unique_ptr<FilterAst> ast;
+        if (filters && filters[q]) {
+            ast = ParseFilter(filters[q], ok);
+            if (!ok) {
+                frk::SetError("filter %u does not parse: %s", q, filters[q]);
+                return FRK_ERR_INVALID;
+            }
+        }
+        trees[q] = BuildQueryNode(ast.get(), posting_field);
+        searcher.AddQuery(trees[q].get(), queries + static_cast<size_t>(q) * dim, dim, ks[q]);
+    }
+    if (opts) {
+        SearchOptions o;
+        o.dtype = opts->dtype;
+        o.rescoreFactor = opts->rescore_factor;
+        o.probe = opts->probe != 0;
+        o.probeStep = opts->probe_step;
+        o.floors = opts->floors;
+        o.candCap = opts->cand_cap;
+        searcher.SetOptions(o);
+    }
+    std::vector<AggrDeviceResult> dev;
+    if (!searcher.BatchSearchDevice(vector_field, false, dev)) return FRK_ERR_INVALID;
+    std::vector<std::shared_ptr<FullRecallResult>> res;
+    if (!searcher.BatchSearchExtract(dev, res, false)) return FRK_ERR_INVALID;
+    for (uint32_t q = 0; q < n_queries; q++) {
+        const auto& r = *res[q];
+        out_counts[q] = static_cast<uint32_t>(r.docIds.size());
+        for (size_t i = 0; i < r.docIds.size(); i++) {
+            out_ids[static_cast<size_t>(q) * k_stride + i] = r.docIds[i];
+            if (out_scores) out_scores[static_cast<size_t>(q) * k_stride + i] = r.scores[i];
+        }
+        if (out_passing) out_passing[q] = dev[q].passing;
+    }
+    if (stats) {
+        const BatchStats& s = searcher.LastStats();
+        stats->ms_filter = s.ms_filter;
+        stats->ms_probe = s.ms_probe;
+        stats->ms_score = s.ms_score;
+        stats->ms_select = s.ms_select;
+        stats->ms_rescore = s.ms_rescore;
+        stats->ms_total = s.ms_total;
+        stats->candidates = s.candidates;
+        stats->reruns = s.reruns;
+        stats->overflowed = s.overflowed;
+        stats->probe_step = s.probeStep;
+        stats->cand_cap = s.cap;
+    }
+    return FRK_OK;
+}
+
+}  // extern "C"

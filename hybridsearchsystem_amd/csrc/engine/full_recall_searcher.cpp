@@ -1,0 +1,435 @@
+// full_recall_searcher.cpp -- the batch pipeline (see full_recall_searcher.h).
+#include "full_recall_searcher.h"
+
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+#include <mutex>
+
+#include "log.h"
+
+namespace NpuRetrieval {
+
+// ------------------------------------------------------------------------------------------ workspace
+// Device and pinned buffers reused across batches: grown to the high-water mark and never freed while
+// the process runs (the reference's GmMemoryManager pools, searcher/runtime/gm_memory_manager.cpp:9-75,
+// without per-call malloc/free). One workspace per concurrent searcher, each with its own stream.
+class SearchWorkspace {
+   public:
+    struct Buf {
+        void* p = nullptr;
+        size_t n = 0;
+        bool pinned = false;
+        bool Ensure(size_t bytes) {
+            if (bytes <= n) return true;
+            size_t want = std::max(bytes, n + n / 4);
+            want = (want + 4095) / 4096 * 4096;
+            if (p) pinned ? (void)hipHostFree(p) : (void)hipFree(p);
+            p = nullptr;
+            n = 0;
+            hipError_t e = pinned ? hipHostMalloc(&p, want, hipHostMallocDefault) : hipMalloc(&p, want);
+            if (e != hipSuccess) {
+                p = nullptr;
+                FRK_LOG_ERROR("workspace allocation of " << want << " B failed");
+                return false;
+            }
+            n = want;
+            return true;
+        }
+        template <typename T>
+        T* As() const {
+            return static_cast<T*>(p);
+        }
+    };
+
+    explicit SearchWorkspace(int32_t device) : m_device(device) {
+        (void)hipSetDevice(device);
+        (void)hipStreamCreateWithFlags(&stream, hipStreamNonBlocking);
+        for (auto& e : ev) (void)hipEventCreate(&e);
+        hbatch.pinned = hout.pinned = true;
+    }
+    ~SearchWorkspace() {
+        (void)hipSetDevice(m_device);
+        for (Buf* b : {&batch, &bitmaps, &bmasks, &counters, &cand, &probe, &lp, &rescored, &out, &hbatch, &hout})
+            if (b->p) b->pinned ? (void)hipHostFree(b->p) : (void)hipFree(b->p);
+        for (auto& e : ev) (void)hipEventDestroy(e);
+        (void)hipStreamDestroy(stream);
+    }
+
+    static std::unique_ptr<SearchWorkspace> Acquire(int32_t device) {
+        std::lock_guard<std::mutex> g(PoolMutex());
+        auto& pool = Pool();
+        for (auto it = pool.begin(); it != pool.end(); ++it) {
+            if ((*it)->m_device == device) {
+                auto w = std::move(*it);
+                pool.erase(it);
+                return w;
+            }
+        }
+        return std::make_unique<SearchWorkspace>(device);
+    }
+    static void Release(std::unique_ptr<SearchWorkspace> w) {
+        std::lock_guard<std::mutex> g(PoolMutex());
+        Pool().push_back(std::move(w));
+    }
+
+    hipStream_t stream = nullptr;
+    hipEvent_t ev[8] = {};
+    Buf batch, bitmaps, bmasks, counters, cand, probe, lp, rescored, out, hbatch, hout;
+
+   private:
+    static std::mutex& PoolMutex() {
+        static std::mutex m;
+        return m;
+    }
+    static std::vector<std::unique_ptr<SearchWorkspace>>& Pool() {
+        static auto* p = new std::vector<std::unique_ptr<SearchWorkspace>>();  // lives until exit
+        return *p;
+    }
+    int32_t m_device;
+};
+
+// ------------------------------------------------------------------------------------------ searcher
+FullRecallSearcher::FullRecallSearcher(int32_t deviceId, std::shared_ptr<DataTable> dataTable)
+    : m_deviceId(deviceId), m_table(std::move(dataTable)) {}
+
+FullRecallSearcher::~FullRecallSearcher() = default;
+
+void FullRecallSearcher::Clear() {
+    m_trees.clear();
+    m_queries.clear();
+    m_topKs.clear();
+    m_dim = 0;
+}
+
+void FullRecallSearcher::AddQuery(QueryNode* queryTree, const std::vector<float>& query, uint32_t topK) {
+    AddQuery(queryTree, query.data(), static_cast<uint32_t>(query.size()), topK);
+}
+
+void FullRecallSearcher::AddQuery(QueryNode* queryTree, const float* query, uint32_t dim, uint32_t topK) {
+    if (m_dim == 0) m_dim = dim;
+    m_trees.push_back(queryTree);
+    const size_t at = m_queries.size();
+    m_queries.resize(at + m_dim, 0.0f);
+    std::memcpy(m_queries.data() + at, query, sizeof(float) * std::min(dim, m_dim));
+    m_topKs.push_back(topK);
+}
+
+bool FullRecallSearcher::BatchSearch(std::vector<std::shared_ptr<FullRecallResult>>& results,
+                                     const std::string& vectorFieldName, bool isMultiShard) {
+    std::vector<AggrDeviceResult> dev;
+    if (!BatchSearchDevice(vectorFieldName, isMultiShard, dev)) return false;
+    return BatchSearchExtract(dev, results, isMultiShard);
+}
+
+bool FullRecallSearcher::BatchSearchDevice(const std::string& vectorFieldName, bool,
+                                           std::vector<AggrDeviceResult>& deviceResults) {
+    if (!m_table) {
+        FRK_LOG_ERROR("searcher has no data table");
+        return false;
+    }
+    VectorFieldData* vf = nullptr;
+    if (!m_table->GetVectorFieldData(vectorFieldName, vf)) {
+        FRK_LOG_ERROR("vector field not in the index: " << vectorFieldName);
+        return false;
+    }
+    if (!m_topKs.empty() && m_dim != vf->Dimension()) {
+        FRK_LOG_ERROR("query dimension " << m_dim << " != field dimension " << vf->Dimension());
+        return false;
+    }
+    if (hipSetDevice(m_deviceId) != hipSuccess) return false;
+    return RunDevice(*vf, deviceResults);
+}
+
+bool FullRecallSearcher::BatchSearchExtract(std::vector<AggrDeviceResult>& dev,
+                                            std::vector<std::shared_ptr<FullRecallResult>>& results,
+                                            bool isMultiShard) {
+    const DocIdMapping* map = m_table->GetDocIdMapping();
+    results.clear();
+    results.reserve(dev.size());
+    for (const AggrDeviceResult& d : dev) {
+        auto r = std::make_shared<FullRecallResult>();
+        r->isMultiShard = isMultiShard;
+        for (uint64_t key : d.keys) {
+            const uint32_t ldoc = 0xFFFFFFFFu - static_cast<uint32_t>(key);
+            const uint32_t ord = static_cast<uint32_t>(key >> 32);
+            const uint32_t bits = (ord & 0x80000000u) ? (ord & 0x7FFFFFFFu) : ~ord;
+            float score;
+            std::memcpy(&score, &bits, 4);
+            uint64_t gdoc = ldoc;
+            if (map && !map->GetGDocId(ldoc, gdoc)) {
+                FRK_LOG_ERROR("no global id for local doc " << ldoc);
+                return false;
+            }
+            if (isMultiShard) {
+                r->scoreWithIndexes.emplace_back(score, ldoc);
+                r->scoreWithIndexes.back().id = gdoc;
+            } else {
+                r->docIds.push_back(static_cast<int64_t>(gdoc));
+                r->scores.push_back(score);
+            }
+        }
+        results.push_back(std::move(r));
+    }
+    return true;
+}
+
+namespace {
+template <typename T>
+size_t Pad(size_t n) {
+    return (n * sizeof(T) + 255) / 256 * 256;
+}
+}  // namespace
+
+bool FullRecallSearcher::RunDevice(const VectorFieldData& vf, std::vector<AggrDeviceResult>& out) {
+    const uint32_t Q = static_cast<uint32_t>(m_topKs.size());
+    out.assign(Q, AggrDeviceResult());
+    m_stats = BatchStats();
+    m_passing.assign(Q, 0);
+    if (Q == 0) return true;
+    const uint32_t dtype = m_options.dtype;
+    const frk_doc_store* store = vf.Store(dtype);
+    const frk_doc_store* f32 = vf.Store(FRK_DTYPE_F32);
+    if (!store || !f32) {
+        FRK_LOG_ERROR("vector store for dtype " << dtype << " was not built for field " << vf.Name());
+        return false;
+    }
+    const uint32_t nDocs = m_table->GetDocNum();
+    const uint32_t dnps = m_table->GetDocNumPerSegment();
+    const uint32_t segs = m_table->GetSegmentNum();
+    if (dnps % 64 != 0) {
+        FRK_LOG_ERROR("docNumPerSegment " << dnps << " is not a multiple of 64 (required by the device filter)");
+        return false;
+    }
+    const uint64_t maskStride = static_cast<uint64_t>(segs) * (dnps / 64);
+    const bool lowp = dtype != FRK_DTYPE_F32;
+    const uint32_t m = lowp ? std::max(1u, m_options.rescoreFactor) : 1;
+    uint32_t kmax = 1;
+    for (uint32_t k : m_topKs) kmax = std::max(kmax, k);
+    if (kmax * m > FRK_SELECT_MAX_K) {
+        FRK_LOG_ERROR("topK " << kmax << " x rescore factor " << m << " exceeds " << FRK_SELECT_MAX_K);
+        return false;
+    }
+    const uint32_t kSel = kmax * m;  // keys selected on the main pass
+
+    // ---- compile the filters
+    std::vector<frk_filter_query> descs(Q);
+    std::vector<uint32_t> expr;
+    std::vector<uint64_t> table;
+    for (uint32_t q = 0; q < Q; q++) {
+        FilterExpr fe;
+        if (!PrepareFilterExpr(*m_table, m_trees[q], fe)) return false;
+        frk_filter_query& d = descs[q];
+        d.expr_off = static_cast<uint32_t>(expr.size());
+        d.table_off = static_cast<uint32_t>(table.size());
+        d.flags = fe.emptyTree ? FRK_FILTER_MATCH_ALL : 0;
+        d.expr_len = fe.emptyTree ? 0 : static_cast<uint32_t>(fe.postExpr.size());
+        d.n_postings = fe.emptyTree ? 0 : static_cast<uint32_t>(fe.postingDeviceAddrs.size());
+        d.op_num = fe.opNum;
+        if (fe.emptyTree) continue;
+        expr.insert(expr.end(), fe.postExpr.begin(), fe.postExpr.end());
+        for (uint32_t s = 0; s < segs; s++)
+            for (const auto* addrs : fe.postingDeviceAddrs) table.push_back((*addrs)[s]);
+    }
+    if (table.size() > 0xFFFFFFFFull || expr.size() > 0xFFFFFFFFull) {
+        FRK_LOG_ERROR("filter tables too large for one batch");
+        return false;
+    }
+
+    // ---- pack the batch: descriptors | expr | table | queries (per store dim) | k | k*m | floors
+    const uint32_t dpMain = store->dim, dpF32 = f32->dim;
+    const size_t oDesc = 0;
+    const size_t oExpr = oDesc + Pad<frk_filter_query>(Q);
+    const size_t oTab = oExpr + Pad<uint32_t>(expr.size() + 1);
+    const size_t oQMain = oTab + Pad<uint64_t>(table.size() + 1);
+    const size_t oQF32 = oQMain + Pad<float>(static_cast<size_t>(Q) * dpMain);
+    const size_t oK = oQF32 + (lowp ? Pad<float>(static_cast<size_t>(Q) * dpF32) : 0);
+    const size_t oKm = oK + Pad<uint32_t>(Q);
+    const size_t oFloor = oKm + Pad<uint32_t>(Q);
+    const size_t batchBytes = oFloor + Pad<float>(Q);
+    std::unique_ptr<SearchWorkspace> ws = SearchWorkspace::Acquire(m_deviceId);
+    struct Releaser {
+        std::unique_ptr<SearchWorkspace>& w;
+        ~Releaser() {
+            if (w) SearchWorkspace::Release(std::move(w));
+        }
+    } releaser{ws};
+    if (!ws->hbatch.Ensure(batchBytes) || !ws->batch.Ensure(batchBytes)) return false;
+    uint8_t* hb = ws->hbatch.As<uint8_t>();
+    std::memcpy(hb + oDesc, descs.data(), sizeof(frk_filter_query) * Q);
+    std::memcpy(hb + oExpr, expr.data(), sizeof(uint32_t) * expr.size());
+    std::memcpy(hb + oTab, table.data(), sizeof(uint64_t) * table.size());
+    auto putQueries = [&](size_t off, uint32_t dp) {
+        float* dst = reinterpret_cast<float*>(hb + off);
+        std::memset(dst, 0, sizeof(float) * Q * dp);
+        for (uint32_t q = 0; q < Q; q++) std::memcpy(dst + static_cast<size_t>(q) * dp, &m_queries[q * m_dim], 4 * m_dim);
+    };
+    putQueries(oQMain, dpMain);
+    if (lowp) putQueries(oQF32, dpF32);
+    uint32_t* hk = reinterpret_cast<uint32_t*>(hb + oK);
+    uint32_t* hkm = reinterpret_cast<uint32_t*>(hb + oKm);
+    for (uint32_t q = 0; q < Q; q++) {
+        hk[q] = m_topKs[q];
+        hkm[q] = m_topKs[q] * m;
+    }
+    const bool callerFloor = m_options.floors != nullptr;
+    if (callerFloor) std::memcpy(hb + oFloor, m_options.floors, sizeof(float) * Q);
+
+    // ---- device buffers
+    const uint64_t nBlocks = (static_cast<uint64_t>(nDocs) + 31) / 32;
+    // Probe sample: every probeStep-th doc block, reduced to one max per (query, block). Automatic step:
+    // keep >= 64 sampled blocks per selected key so the floor is tight, sample at most every 64th block.
+    uint32_t probeStep = m_options.probeStep;
+    const bool autoStep = probeStep == 0;
+    if (autoStep) probeStep = static_cast<uint32_t>(std::min<uint64_t>(64, std::max<uint64_t>(1, nBlocks / (64ull * kSel))));
+    // a query whose k*m exceeds the sampled blocks gets a -inf floor (no pruning), which stays exact
+    uint32_t kSelMin = kSel;
+    for (uint32_t k : m_topKs) kSelMin = std::min(kSelMin, k * m);
+    const bool useProbe = m_options.probe && !callerFloor && probeStep > 1 && nBlocks / probeStep >= kSelMin;
+    uint32_t cap = m_options.candCap;
+    if (cap == 0) {
+        cap = useProbe ? static_cast<uint32_t>(std::max<uint64_t>(65536, 16ull * kSel * probeStep))
+                       : static_cast<uint32_t>(std::min<uint64_t>(nDocs, std::max<uint64_t>(65536, 64ull * kSel)));
+        if (callerFloor) cap = static_cast<uint32_t>(std::max<uint64_t>(cap, 65536));
+        cap = std::min<uint32_t>(cap, std::max<uint32_t>(nDocs, 1));
+    }
+    const uint64_t probeBlocks = useProbe ? (nBlocks + probeStep - 1) / probeStep : 0;
+    m_stats.probeStep = useProbe ? probeStep : 0;
+
+    // counters: [filter counts Q][cand counts Q][status 64][out counts Q][lp counts Q]
+    const size_t cFilter = 0, cCand = Q, cStatus = 2 * Q, cOut = 2 * Q + 64, cLp = 3 * Q + 64;
+    const size_t nCounters = 4 * Q + 64;
+    float* dKth = nullptr;
+    const uint32_t qStride = (Q + 31) / 32 * 32;  // block-major masks: one row of qStride u32 per doc block
+    if (!ws->bitmaps.Ensure(sizeof(uint64_t) * Q * maskStride) || !ws->bmasks.Ensure(sizeof(uint32_t) * qStride * nBlocks) || !ws->counters.Ensure(sizeof(uint32_t) * nCounters + 4 * Q) ||
+        !ws->cand.Ensure(sizeof(uint64_t) * Q * cap) || !ws->out.Ensure(sizeof(uint64_t) * Q * kmax) ||
+        (useProbe && !ws->probe.Ensure(sizeof(float) * Q * probeBlocks)) ||
+        (lowp && (!ws->lp.Ensure(sizeof(uint64_t) * Q * kSel) || !ws->rescored.Ensure(sizeof(uint64_t) * Q * kSel))) ||
+        !ws->hout.Ensure(sizeof(uint64_t) * Q * kmax + sizeof(uint32_t) * nCounters))
+        return false;
+    uint32_t* dCnt = ws->counters.As<uint32_t>();
+    dKth = reinterpret_cast<float*>(dCnt + nCounters);
+    uint8_t* db = ws->batch.As<uint8_t>();
+    hipStream_t s = ws->stream;
+    auto fail = [&](const char* what) {
+        FRK_LOG_ERROR(what << ": " << frk_last_error());
+        (void)hipStreamSynchronize(s);
+        return false;
+    };
+
+    // ---- the pipeline
+    (void)hipEventRecord(ws->ev[0], s);
+    if (hipMemcpyAsync(db, hb, batchBytes, hipMemcpyHostToDevice, s) != hipSuccess) return fail("batch H2D");
+    if (hipMemsetAsync(dCnt, 0, sizeof(uint32_t) * nCounters, s) != hipSuccess) return fail("counter reset");
+    const float* dQMain = reinterpret_cast<const float*>(db + oQMain);
+    const float* dQF32 = lowp ? reinterpret_cast<const float*>(db + oQF32) : dQMain;
+    const uint32_t* dK = reinterpret_cast<const uint32_t*>(db + oK);
+    const uint32_t* dKm = reinterpret_cast<const uint32_t*>(db + oKm);
+    uint64_t* dMask = ws->bitmaps.As<uint64_t>();
+    if (frk_filter_eval(reinterpret_cast<const frk_filter_query*>(db + oDesc), Q,
+                        reinterpret_cast<const uint32_t*>(db + oExpr), reinterpret_cast<const uint64_t*>(db + oTab),
+                        segs, dnps, nDocs, dMask, maskStride, dCnt + cFilter, dCnt + cStatus, s) != 0)
+        return fail("filter");
+    uint32_t* dBMask = ws->bmasks.As<uint32_t>();
+    if (frk_masks_to_blocks(dMask, maskStride, Q, nDocs, dBMask, qStride, s) != 0) return fail("mask transpose");
+    (void)hipEventRecord(ws->ev[1], s);
+    const float* dFloor = callerFloor ? reinterpret_cast<const float*>(db + oFloor) : nullptr;
+    if (useProbe) {
+        // the k-th (k*m on low precision) largest block maximum of the sample is the floor
+        if (frk_score_blockmax(store, dQMain, Q, dBMask, qStride, 0, probeStep, probeBlocks, ws->probe.As<float>(),
+                               static_cast<uint32_t>(probeBlocks), s) != 0)
+            return fail("probe score");
+        if (frk_kth_largest(ws->probe.As<float>(), static_cast<uint32_t>(probeBlocks),
+                            static_cast<uint32_t>(probeBlocks), Q, dKm, 0, dKth, s) != 0)
+            return fail("probe select");
+        dFloor = dKth;
+    }
+    (void)hipEventRecord(ws->ev[2], s);
+    if (frk_score_select(store, dQMain, Q, dBMask, qStride, dFloor, 0, 1, nBlocks, ws->cand.As<uint64_t>(), cap,
+                         dCnt + cCand, s) != 0)
+        return fail("score");
+    (void)hipEventRecord(ws->ev[3], s);
+    if (!lowp) {
+        if (frk_select_topk(ws->cand.As<uint64_t>(), cap, dCnt + cCand, Q, dK, 0, ws->out.As<uint64_t>(), kmax,
+                            dCnt + cOut, nullptr, s) != 0)
+            return fail("select");
+        (void)hipEventRecord(ws->ev[4], s);
+        (void)hipEventRecord(ws->ev[5], s);
+    } else {
+        if (frk_select_topk(ws->cand.As<uint64_t>(), cap, dCnt + cCand, Q, dKm, 0, ws->lp.As<uint64_t>(), kSel,
+                            dCnt + cLp, nullptr, s) != 0)
+            return fail("select k*m");
+        (void)hipEventRecord(ws->ev[4], s);
+        if (frk_rescore(f32, dQF32, Q, ws->lp.As<uint64_t>(), kSel, dCnt + cLp, ws->rescored.As<uint64_t>(), s) != 0)
+            return fail("rescore");
+        if (frk_select_topk(ws->rescored.As<uint64_t>(), kSel, dCnt + cLp, Q, dK, 0, ws->out.As<uint64_t>(), kmax,
+                            dCnt + cOut, nullptr, s) != 0)
+            return fail("select after rescore");
+        (void)hipEventRecord(ws->ev[5], s);
+    }
+    uint8_t* ho = ws->hout.As<uint8_t>();
+    if (hipMemcpyAsync(ho, ws->out.p, sizeof(uint64_t) * Q * kmax, hipMemcpyDeviceToHost, s) != hipSuccess ||
+        hipMemcpyAsync(ho + sizeof(uint64_t) * Q * kmax, dCnt, sizeof(uint32_t) * nCounters, hipMemcpyDeviceToHost, s) !=
+            hipSuccess)
+        return fail("result D2H");
+    (void)hipEventRecord(ws->ev[6], s);
+    if (hipStreamSynchronize(s) != hipSuccess) return fail("batch synchronize");
+
+    const uint64_t* hKeys = reinterpret_cast<const uint64_t*>(ho);
+    const uint32_t* hCnt = reinterpret_cast<const uint32_t*>(ho + sizeof(uint64_t) * Q * kmax);
+    if (hCnt[cStatus] != 0) {
+        FRK_LOG_ERROR("filter kernel rejected a program (status " << hCnt[cStatus] << ")");
+        return false;
+    }
+    auto ms = [&](int a, int b) {
+        float t = 0;
+        (void)hipEventElapsedTime(&t, ws->ev[a], ws->ev[b]);
+        return t;
+    };
+    m_stats.ms_filter = ms(0, 1);
+    m_stats.ms_probe = ms(1, 2);
+    m_stats.ms_score = ms(2, 3);
+    m_stats.ms_select = ms(3, 4);
+    m_stats.ms_rescore = ms(4, 5);
+    m_stats.ms_total = ms(0, 6);
+    m_stats.cap = cap;
+
+    // ---- exactness checks: overflowed candidate lists, and a caller floor that cut below the k-th
+    std::vector<uint32_t> redo;
+    uint32_t needCap = cap;
+    for (uint32_t q = 0; q < Q; q++) {
+        m_passing[q] = hCnt[cFilter + q];
+        m_stats.candidates += hCnt[cCand + q];
+        const uint32_t want = std::min(m_topKs[q] * m, m_passing[q]);
+        const bool overflow = hCnt[cCand + q] > cap;
+        const bool shortFloor = callerFloor && std::min(hCnt[cCand + q], cap) < want;
+        if (overflow) m_stats.overflowed++;
+        if (overflow || shortFloor) {
+            redo.push_back(q);
+            needCap = std::max(needCap, std::min(hCnt[cCand + q], m_passing[q]));
+        }
+    }
+    if (!redo.empty()) {
+        // rerun the batch without the caller floor (probe floor or none) and room for every survivor
+        m_stats.reruns = static_cast<uint32_t>(redo.size());
+        SearchOptions saved = m_options;
+        BatchStats first = m_stats;
+        m_options.floors = nullptr;
+        m_options.candCap = std::max<uint32_t>(needCap, cap);
+        if (callerFloor) m_options.probe = true;
+        const bool ok = RunDevice(vf, out);
+        m_options = saved;
+        first.reruns = static_cast<uint32_t>(redo.size());
+        first.ms_total += m_stats.ms_total;
+        m_stats = first;
+        return ok;
+    }
+    for (uint32_t q = 0; q < Q; q++) {
+        const uint32_t n = hCnt[cOut + q];
+        out[q].keys.assign(hKeys + static_cast<size_t>(q) * kmax, hKeys + static_cast<size_t>(q) * kmax + n);
+        out[q].passing = m_passing[q];
+    }
+    return true;
+}
+
+}  // namespace NpuRetrieval

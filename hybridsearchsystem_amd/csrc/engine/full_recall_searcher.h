@@ -1,0 +1,105 @@
+// full_recall_searcher.h -- the per-batch filtered top-K search with the reference's interface
+// (retrieval/searcher/full_recall_searcher.h:12-70, core/full_recall_result.h:38-51).
+//
+// One batch is one stream-ordered device pipeline with a single synchronisation at the end:
+//   H2D of the packed batch (programs, operand tables, queries, k) -> filter (all queries) ->
+//   [probe: score a strided sample of doc blocks, k-th best per query = the score floor] ->
+//   fused score + mask + floor + compaction over all docs -> exact top-k select ->
+//   [low precision: FP32 rescore of the top k*m, select again] -> D2H of the top-k keys.
+// The reference instead syncs after the scorer, per query after the filter and per query after
+// each aggregation step (SURVEY call stack S2).
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <atomic>
+#include <cstdint>
+#include <memory>
+#include <string>
+#include <vector>
+
+#include "data_table.h"
+#include "query_node.h"
+
+namespace NpuRetrieval {
+
+struct ScoreWithIndex {
+    float score;
+    uint32_t index;  // local doc id
+    uint64_t id{0};  // global doc id
+    ScoreWithIndex(float s, uint32_t i) : score(s), index(i) {}
+};
+
+// Single shard: docIds (gdocid) and scores, best first, min(k, passing docs) long. Multi-shard:
+// scoreWithIndexes {score, ldocid, gdocid} in the same order (result_aggregator.cpp:846-1032).
+struct FullRecallResult {
+    bool isMultiShard = false;
+    std::vector<ScoreWithIndex> scoreWithIndexes;
+    std::vector<float> scores;
+    std::vector<int64_t> docIds;
+};
+
+// What BatchSearchDevice hands to BatchSearchExtract for one query: the sorted top-k keys.
+struct AggrDeviceResult {
+    std::vector<uint64_t> keys;
+    uint32_t passing = 0;  // docs that passed the filter
+};
+
+// Scoring precision and pruning for a batch.
+struct SearchOptions {
+    uint32_t dtype = FRK_DTYPE_F32;  // store scored on the main pass
+    uint32_t rescoreFactor = 4;      // low precision: FP32 rescoring of the top k * m (m = this)
+    bool probe = true;               // sampled k-th best score as the pruning floor (exact)
+    uint32_t probeStep = 0;          // sample every probeStep-th doc block; 0 = chosen from k and N
+    const float* floors = nullptr;   // optional caller floor per query (e.g. a learned bound); the
+                                     // batch reruns unpruned for any query left with < k survivors
+    uint32_t candCap = 0;            // candidate capacity per query; 0 = automatic
+};
+
+struct BatchStats {
+    float ms_filter = 0, ms_probe = 0, ms_score = 0, ms_select = 0, ms_rescore = 0, ms_total = 0;
+    uint64_t candidates = 0;     // survivors written on the main pass
+    uint32_t reruns = 0;         // queries rerun (overflow or an unsafe caller floor)
+    uint32_t overflowed = 0;
+    uint32_t probeStep = 0;
+    uint32_t cap = 0;
+};
+
+class SearchWorkspace;
+
+class FullRecallSearcher {
+   public:
+    FullRecallSearcher(int32_t deviceId, std::shared_ptr<DataTable> dataTable);
+    ~FullRecallSearcher();
+
+    void AddQuery(QueryNode* queryTree, const std::vector<float>& query, uint32_t topK);
+    void AddQuery(QueryNode* queryTree, const float* query, uint32_t dim, uint32_t topK);
+    void SetOptions(const SearchOptions& o) { m_options = o; }
+    const BatchStats& LastStats() const { return m_stats; }
+    void Clear();
+
+    bool BatchSearch(std::vector<std::shared_ptr<FullRecallResult>>& results, const std::string& vectorFieldName,
+                     bool isMultiShard);
+    bool BatchSearchDevice(const std::string& vectorFieldName, bool isMultiShard,
+                           std::vector<AggrDeviceResult>& deviceResults);
+    bool BatchSearchExtract(std::vector<AggrDeviceResult>& deviceResults,
+                            std::vector<std::shared_ptr<FullRecallResult>>& results, bool isMultiShard);
+
+    // Per-query filter pass counts of the last batch (for diagnostics and the learned bound).
+    const std::vector<uint32_t>& LastPassing() const { return m_passing; }
+
+   private:
+    bool RunDevice(const VectorFieldData& vf, std::vector<AggrDeviceResult>& out);
+
+    int32_t m_deviceId;
+    std::shared_ptr<DataTable> m_table;
+    std::vector<QueryNode*> m_trees;
+    std::vector<float> m_queries;  // [n][dim]
+    std::vector<uint32_t> m_topKs;
+    uint32_t m_dim = 0;
+    SearchOptions m_options;
+    BatchStats m_stats;
+    std::vector<uint32_t> m_passing;
+};
+
+}  // namespace NpuRetrieval

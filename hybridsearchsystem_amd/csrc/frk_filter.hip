@@ -9,8 +9,8 @@
 //
 // Work split: one workgroup of 256 threads per (query, segment, 256-word chunk); each thread owns one
 // u64 word (64 docs). Phase 1 stages every non-dense operand of the chunk into an LDS slot:
-// SPARSE_BITMAP (1) and SPARSE_PACKED (4) units are located by binary search (their units are
-// ascending, posting_serializer.cpp:142-186) and scattered with LDS atomic ORs, SPARSE_IDLIST (3)
+// SPARSE_BITMAP (1) and SPARSE_PACKED (4) units of the chunk are located by a two-round cooperative
+// search (their units are ascending, posting_serializer.cpp:142-186) and scattered with LDS atomic ORs, SPARSE_IDLIST (3)
 // ids are filtered and scattered, DENSE_BYTEMAP (2) bytes are packed to bits. Phase 2 runs the
 // program per word with the stack in registers; DENSE_BITMAP (0) operands are read straight from
 // HBM with coalesced 8-byte loads.
@@ -69,6 +69,38 @@ __device__ inline void UnitAt(const PostingHead& h, uint32_t k, uint32_t i, uint
         unit = words[i] >> 1;  // byte offset of the u16 unit in the dense bitmap
         mask = reinterpret_cast<const uint16_t*>(h.payload + 4ull * k)[i];
     }
+}
+
+// Both lower bounds (first unit >= a, first unit >= b) of a sorted unit list, found by the whole
+// workgroup in two rounds of parallel probes instead of a dependent binary search per thread: round 1
+// probes every s-th entry (s = ceil(k / 256)) and counts the probes below each target, round 2 probes
+// the s entries of the bracket that count selects. Called by every thread of the workgroup.
+__device__ inline void CoopLowerBounds(const PostingHead& h, uint32_t k, uint32_t a, uint32_t b, uint32_t& lo,
+                                       uint32_t& hi) {
+    const uint32_t t = threadIdx.x;
+    const uint32_t s = (k + kChunkWords - 1) / kChunkWords;  // <= 32 for a segment of 8192 units
+    if (s > kChunkWords) {  // more units than two rounds cover (segments beyond 1M docs): per-thread search
+        lo = LowerBoundUnit(h, k, a);
+        hi = LowerBoundUnit(h, k, b);
+        return;
+    }
+    const uint32_t* words = reinterpret_cast<const uint32_t*>(h.payload);
+    auto unit_at = [&](uint32_t i) { return h.layout == 4 ? (words[i] >> 16) : (words[i] >> 1); };
+    const uint32_t p1 = t * s;
+    uint32_t u1 = 0xFFFFFFFFu;
+    if (s && p1 < k) u1 = unit_at(p1);
+    const uint32_t ca = __syncthreads_count(p1 < k && u1 < a);
+    const uint32_t cb = __syncthreads_count(p1 < k && u1 < b);
+    // the bound lies in (p_{c-1}, p_c]: probe the up to s - 1 entries strictly between
+    auto round2 = [&](uint32_t c, uint32_t target) {
+        if (c == 0) return __syncthreads_count(false) * 0u;
+        const uint32_t base = (c - 1) * s + 1;
+        const uint32_t p2 = base + t;
+        const bool below = t + 1 < s && p2 < k && unit_at(p2) < target;
+        return base + __syncthreads_count(below);
+    };
+    lo = round2(ca, a);
+    hi = round2(cb, b);
 }
 
 // One operand's 64 bits at segment word w, read directly (dense and the per-thread fallback).
@@ -182,8 +214,8 @@ __global__ __launch_bounds__(kChunkWords) void FilterKernel(const frk_filter_que
             __syncthreads();
             if (h.layout == 1 || h.layout == 4) {
                 const uint32_t k = h.len / (h.layout == 4 ? 4u : 6u);
-                const uint32_t lo = LowerBoundUnit(h, k, 4 * w0);
-                const uint32_t hi = LowerBoundUnit(h, k, 4 * (w0 + kChunkWords));
+                uint32_t lo, hi;
+                CoopLowerBounds(h, k, 4 * w0, 4 * (w0 + kChunkWords), lo, hi);
                 for (uint32_t i = lo + t; i < hi; i += kChunkWords) {
                     uint32_t unit, mask;
                     UnitAt(h, k, i, unit, mask);

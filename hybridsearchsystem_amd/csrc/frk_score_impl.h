@@ -1,0 +1,499 @@
+// frk_score_impl.h -- fused inner-product scorer on MFMA with the filter mask, a per-query score floor and
+// survivor compaction in the epilogue. No score matrix is written.
+//
+// The reference computes S = Q x D^T on the Cube unit into a full [Q][docs] FP32 matrix
+// (engine/retrieval/device/scorer/kernel_vector_mmad_op.h:21-199) and a second kernel re-reads it with
+// the filter bitmap to compact survivors (device/aggregator/kernel_aggregator.h:22-182). Here one pass
+// does both: a wave computes a 32-doc block against the whole query tile held in LDS, then each
+// accumulator row is tested against (mask bit, floor), ballot-ed, and the passing (score, doc) pairs
+// are appended to per-query candidate lists at offsets from one atomic per (wave, query) and the
+// prefix count of the ballot.
+//
+// Layout (frk_capi.h, "Doc store"): a 32-doc x 16-byte-per-lane chunk tile is exactly one wave load of
+// 1 KiB and exactly the B operand of one MFMA step group:
+//   f32:  4 x v_mfma_f32_32x32x2_f32   per chunk (lane halves carry dims 8c+e and 8c+4+e)
+//   bf16: 1 x v_mfma_f32_32x32x16_bf16 per chunk
+//   fp8:  1 x v_mfma_scale_f32_32x32x64_f8f6f4 per two chunks (scale exponents 127 = x1.0)
+// The query tile uses the same tiling (rows = queries) and lives in LDS for the whole launch.
+// Accumulator element r of lane l holds C[i][j] with j = l % 32 (the doc) and
+// i = 8 (r / 4) + 4 (l / 32) + r % 4 (the query row).
+#include <math.h>
+
+#pragma once
+#include <utility>
+
+#include "frk_common.h"
+
+namespace frk {
+
+typedef float floatx16 __attribute__((ext_vector_type(16)));
+typedef float floatx4 __attribute__((ext_vector_type(4)));
+typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
+typedef int intx8 __attribute__((ext_vector_type(8)));
+typedef unsigned int u32x4 __attribute__((ext_vector_type(4)));
+
+constexpr int kThreads = 256;
+constexpr int kWaves = kThreads / 64;
+#ifndef FRK_SCORE_MIN_BLOCKS
+#define FRK_SCORE_MIN_BLOCKS 1  // workgroups per CU the register budget is sized for
+#endif
+
+template <int DT>
+struct Traits;
+template <>
+struct Traits<FRK_DTYPE_F32> {
+    static constexpr int kElemsPerLane = 4;  // per 16-byte chunk and lane half
+};
+template <>
+struct Traits<FRK_DTYPE_BF16> {
+    static constexpr int kElemsPerLane = 8;
+};
+template <>
+struct Traits<FRK_DTYPE_FP8E4M3> {
+    static constexpr int kElemsPerLane = 16;
+};
+
+// One chunk (or, for fp8, one chunk pair) of MFMA work for one 32-query row block.
+template <int DT>
+__device__ __forceinline__ void MfmaChunk(const u32x4& a, const u32x4& b, floatx16& acc);
+
+template <>
+__device__ __forceinline__ void MfmaChunk<FRK_DTYPE_F32>(const u32x4& a, const u32x4& b, floatx16& acc) {
+    floatx4 av = __builtin_bit_cast(floatx4, a);
+    floatx4 bv = __builtin_bit_cast(floatx4, b);
+    acc = __builtin_amdgcn_mfma_f32_32x32x2f32(av[0], bv[0], acc, 0, 0, 0);
+    acc = __builtin_amdgcn_mfma_f32_32x32x2f32(av[1], bv[1], acc, 0, 0, 0);
+    acc = __builtin_amdgcn_mfma_f32_32x32x2f32(av[2], bv[2], acc, 0, 0, 0);
+    acc = __builtin_amdgcn_mfma_f32_32x32x2f32(av[3], bv[3], acc, 0, 0, 0);
+}
+
+template <>
+__device__ __forceinline__ void MfmaChunk<FRK_DTYPE_BF16>(const u32x4& a, const u32x4& b, floatx16& acc) {
+    acc = __builtin_amdgcn_mfma_f32_32x32x16_bf16(__builtin_bit_cast(bf16x8, a), __builtin_bit_cast(bf16x8, b), acc,
+                                                  0, 0, 0);
+}
+
+__device__ __forceinline__ void MfmaChunkPairFp8(const u32x4& a0, const u32x4& a1, const u32x4& b0, const u32x4& b1,
+                                        floatx16& acc) {
+    intx8 av = {static_cast<int>(a0[0]), static_cast<int>(a0[1]), static_cast<int>(a0[2]), static_cast<int>(a0[3]),
+                static_cast<int>(a1[0]), static_cast<int>(a1[1]), static_cast<int>(a1[2]), static_cast<int>(a1[3])};
+    intx8 bv = {static_cast<int>(b0[0]), static_cast<int>(b0[1]), static_cast<int>(b0[2]), static_cast<int>(b0[3]),
+                static_cast<int>(b1[0]), static_cast<int>(b1[1]), static_cast<int>(b1[2]), static_cast<int>(b1[3])};
+    // formats: 0 = FP8 E4M3 for both operands; scales 127 = 2^0
+    acc = __builtin_amdgcn_mfma_scale_f32_32x32x64_f8f6f4(av, bv, acc, 0, 0, 0, 127, 0, 127);
+}
+
+struct ScoreArgs {
+    const u32x4* tiles;
+    const float* doc_scale;
+    uint64_t n_docs;
+    const float* queries;
+    uint32_t n_queries;
+    uint32_t dim;  // padded dim of the store
+    uint32_t q_dim;  // row length of the query matrix (== dim)
+    const uint32_t* bmask;    // block-major filter masks: bmask[block * bmask_stride + q], bit j = doc 32 block + j
+    uint32_t bmask_stride;    // >= n_queries; NULL bmask = every doc passes
+    const float* floor;
+    uint64_t block_begin, block_step, block_count;
+    uint64_t* cand;
+    uint32_t cap;
+    uint32_t* counts;
+};
+
+// Writes the query tile into LDS in the doc tiling, converted to the store's element type; for fp8 the
+// per-query scale (max|q| / 448) goes to q_scale_lds.
+template <int DT, int CH, int RB>
+__device__ __forceinline__ void StageQueries(const ScoreArgs& A, u32x4* a_lds, float* q_scale_lds, uint32_t q0) {
+    constexpr int E = Traits<DT>::kElemsPerLane;
+    if constexpr (DT == FRK_DTYPE_FP8E4M3) {
+        for (int r = threadIdx.x; r < 32 * RB; r += kThreads) {
+            uint32_t q = q0 + r;
+            float mx = 0.0f;
+            if (q < A.n_queries)
+                for (uint32_t d = 0; d < A.dim; d++) mx = fmaxf(mx, fabsf(A.queries[static_cast<uint64_t>(q) * A.q_dim + d]));
+            q_scale_lds[r] = mx > 0.0f ? mx / 448.0f : 1.0f;
+        }
+        __syncthreads();
+    }
+    for (int idx = threadIdx.x; idx < RB * CH * 64; idx += kThreads) {
+        const int l = idx & 63;
+        const int c = (idx >> 6) % CH;
+        const int rb = idx / (CH * 64);
+        const int i = l & 31, h = l >> 5;
+        const uint32_t q = q0 + 32 * rb + i;
+        const uint32_t d0 = c * 2 * E + h * E;
+        float v[E];
+        for (int e = 0; e < E; e++) {
+            uint32_t d = d0 + e;
+            v[e] = (q < A.n_queries && d < A.dim) ? A.queries[static_cast<uint64_t>(q) * A.q_dim + d] : 0.0f;
+        }
+        u32x4 packed;
+        uint32_t* pw = reinterpret_cast<uint32_t*>(&packed);
+        if constexpr (DT == FRK_DTYPE_F32) {
+            for (int e = 0; e < 4; e++) pw[e] = __builtin_bit_cast(uint32_t, v[e]);
+        } else if constexpr (DT == FRK_DTYPE_BF16) {
+            for (int e = 0; e < 4; e++)
+                pw[e] = static_cast<uint32_t>(F32ToBf16(v[2 * e])) | (static_cast<uint32_t>(F32ToBf16(v[2 * e + 1])) << 16);
+        } else {
+            const float inv = 1.0f / q_scale_lds[32 * rb + i];
+            for (int e = 0; e < 4; e++) {
+                uint32_t w = 0;
+                for (int x = 0; x < 4; x++) w |= static_cast<uint32_t>(F32ToE4M3(v[4 * e + x] * inv)) << (8 * x);
+                pw[e] = w;
+            }
+        }
+        a_lds[idx] = packed;
+    }
+}
+
+// Chunks per pipeline group and groups in flight: a wave streams its doc blocks as a flat sequence of
+// (block, group) steps through a ring of kRing register buffers, so kRing - 1 groups of loads are in
+// flight while one group feeds the MFMAs.
+constexpr int kRing = 3;
+
+template <int CH>
+struct GroupShape {
+    static constexpr int G = CH < 8 ? CH : 8;  // chunks per group
+    static constexpr int kGroups = CH / G;     // groups per doc block
+    static_assert(CH % G == 0, "chunks per doc must be a multiple of the group size");
+};
+
+template <int DT, int CH, int RB>
+__device__ __forceinline__ void ComputeGroup(const u32x4* a_lds, int g, const u32x4 (&buf)[GroupShape<CH>::G],
+                                    floatx16 (&acc)[RB]) {
+    constexpr int G = GroupShape<CH>::G;
+    const int lane = threadIdx.x & 63;
+    if constexpr (DT == FRK_DTYPE_FP8E4M3) {
+#pragma unroll
+        for (int c = 0; c < G; c += 2) {
+#pragma unroll
+            for (int rb = 0; rb < RB; rb++) {
+                const u32x4 a0 = a_lds[(rb * CH + g * G + c) * 64 + lane];
+                const u32x4 a1 = a_lds[(rb * CH + g * G + c + 1) * 64 + lane];
+                MfmaChunkPairFp8(a0, a1, buf[c], buf[c + 1], acc[rb]);
+            }
+        }
+    } else {
+#pragma unroll
+        for (int c = 0; c < G; c++) {
+#pragma unroll
+            for (int rb = 0; rb < RB; rb++) MfmaChunk<DT>(a_lds[(rb * CH + g * G + c) * 64 + lane], buf[c], acc[rb]);
+        }
+    }
+}
+
+// Survivor queue: a wave appends the (key) of every doc that passed its row's floor and filter bit to a
+// wave-private LDS queue, in runs of one (row, lane half) = one query each; the queue is flushed in bulk
+// (one counter atomic per run, all issued before any result is awaited, then the stores), so no global
+// memory access or atomic sits inside the MFMA loop.
+constexpr int kQueueKeys = 512;  // keys per wave
+constexpr int kQueueRuns = 128;  // runs per wave
+
+struct WaveQueue {
+    uint64_t* keys;  // [kQueueKeys]
+    uint64_t* runs;  // [kQueueRuns]: q | start << 32 | count << 48
+    uint32_t nk, nr; // uniform across the wave
+};
+
+__device__ __forceinline__ void WaveSync() {
+    __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+}
+
+__device__ __forceinline__ void FlushQueue(const ScoreArgs& A, WaveQueue& Qu) {
+    const int lane = threadIdx.x & 63;
+    WaveSync();
+    for (uint32_t i0 = 0; i0 < Qu.nr; i0 += 64) {
+        const uint32_t i = i0 + lane;
+        uint64_t run = 0;
+        uint32_t base = 0;
+        if (i < Qu.nr) {
+            run = Qu.runs[i];
+            base = atomicAdd(&A.counts[static_cast<uint32_t>(run)], static_cast<uint32_t>(run >> 48));
+        }
+        if (i < Qu.nr) {
+            const uint32_t q = static_cast<uint32_t>(run);
+            const uint32_t start = static_cast<uint32_t>(run >> 32) & 0xFFFFu;
+            const uint32_t cnt = static_cast<uint32_t>(run >> 48);
+            uint64_t* dst = A.cand + static_cast<uint64_t>(q) * A.cap;
+            for (uint32_t e = 0; e < cnt; e++)
+                if (base + e < A.cap) dst[base + e] = Qu.keys[start + e];
+        }
+    }
+    WaveSync();
+    Qu.nk = 0;
+    Qu.nr = 0;
+}
+
+// The block's filter masks for this wave's row blocks, staged through LDS so each lane can read the
+// 16 rows it holds: lane l fetches query row (l % 32) of each row block (one coalesced 128-byte load).
+template <int RB>
+__device__ __forceinline__ void StageMasks(const ScoreArgs& A, uint32_t q0, uint64_t blk, uint32_t* mask_lds) {
+    const int lane = threadIdx.x & 63;
+#pragma unroll
+    for (int rb = 0; rb < RB; rb++) {
+        const uint32_t q = q0 + 32 * rb + (lane & 31);
+        uint32_t m = 0;
+        if (q < A.n_queries) m = A.bmask ? A.bmask[blk * A.bmask_stride + q] : 0xFFFFFFFFu;
+        if (lane < 32) mask_lds[32 * rb + lane] = m;
+    }
+    WaveSync();
+}
+
+// One row block of the epilogue; rb is a template parameter so the accumulator is always indexed
+// statically (a runtime row-block index would send it to scratch).
+template <int DT, int RB, int rb>
+__device__ __forceinline__ void EpilogueRowBlock(const ScoreArgs& A, const float* q_scale_lds, const float* floor_lds,
+                                        const uint32_t* mask_lds, uint32_t q0, uint64_t doc, bool doc_ok, float dscale,
+                                        const floatx16& accr, WaveQueue& Qu) {
+    const int lane = threadIdx.x & 63;
+    const uint32_t j = lane & 31;
+    const uint32_t half = lane >> 5;
+    {
+        uint32_t hits = 0;
+        float sv[16];
+#pragma unroll
+        for (int g = 0; g < 4; g++) {
+            const int row = 32 * rb + 8 * g + 4 * half;
+            const floatx4 f4 = *reinterpret_cast<const floatx4*>(floor_lds + row);
+            const u32x4 m4 = *reinterpret_cast<const u32x4*>(mask_lds + row);
+#pragma unroll
+            for (int x = 0; x < 4; x++) {
+                float v = accr[4 * g + x];
+                if constexpr (DT == FRK_DTYPE_FP8E4M3) v *= dscale * q_scale_lds[row + x];
+                sv[4 * g + x] = v;
+                hits |= static_cast<uint32_t>(v >= f4[x] && ((m4[x] >> j) & 1u)) << (4 * g + x);
+            }
+        }
+        if (!doc_ok) hits = 0;
+        if (__ballot(hits != 0) == 0) return;
+        // two halves of 8 rows: count a half's keys and runs first, flush if they would not fit, then
+        // append (so the unrolled append loop itself never flushes)
+#pragma unroll
+        for (int hb = 0; hb < 2; hb++) {
+            uint64_t bal[8];
+            uint32_t nkeys = 0, nruns = 0;
+#pragma unroll
+            for (int x = 0; x < 8; x++) {
+                bal[x] = __ballot((hits >> (8 * hb + x)) & 1u);
+                nkeys += static_cast<uint32_t>(__popcll(bal[x]));
+                nruns += (static_cast<uint32_t>(bal[x]) != 0) + (static_cast<uint32_t>(bal[x] >> 32) != 0);
+            }
+            if (nkeys == 0) continue;
+            if (Qu.nk + nkeys > kQueueKeys || Qu.nr + nruns > kQueueRuns) FlushQueue(A, Qu);
+#pragma unroll
+            for (int x = 0; x < 8; x++) {
+                const int r = 8 * hb + x;
+                if (bal[x] == 0) continue;
+                const uint32_t lo = static_cast<uint32_t>(bal[x]), hi = static_cast<uint32_t>(bal[x] >> 32);
+                const uint32_t below = static_cast<uint32_t>(__popcll(bal[x] & ((1ull << lane) - 1ull)));
+                if ((hits >> r) & 1u) Qu.keys[Qu.nk + below] = MakeKey(sv[r], static_cast<uint32_t>(doc));
+                const uint32_t qa = q0 + 32 * rb + 8 * (r / 4) + (r % 4);  // row of lane half 0
+                const uint32_t nlo = static_cast<uint32_t>(__popc(lo)), nhi = static_cast<uint32_t>(__popc(hi));
+                if (lane == 0) {
+                    uint32_t at = Qu.nr;
+                    if (nlo) Qu.runs[at++] = qa | (static_cast<uint64_t>(Qu.nk) << 32) | (static_cast<uint64_t>(nlo) << 48);
+                    if (nhi)
+                        Qu.runs[at] =
+                            (qa + 4) | (static_cast<uint64_t>(Qu.nk + nlo) << 32) | (static_cast<uint64_t>(nhi) << 48);
+                }
+                Qu.nk += nlo + nhi;
+                Qu.nr += (nlo != 0) + (nhi != 0);
+            }
+        }
+    }
+}
+
+template <int DT, int RB, int... I>
+__device__ __forceinline__ void EpilogueAll(const ScoreArgs& A, const float* q_scale_lds, const float* floor_lds,
+                                   const uint32_t* mask_lds, uint32_t q0, uint64_t doc, bool doc_ok, float dscale,
+                                   const floatx16 (&acc)[RB], WaveQueue& Qu, std::integer_sequence<int, I...>) {
+    (EpilogueRowBlock<DT, RB, I>(A, q_scale_lds, floor_lds, mask_lds, q0, doc, doc_ok, dscale, acc[I], Qu), ...);
+}
+
+// Epilogue of one finished 32-doc block: every accumulator row against (floor, filter bit); survivors
+// go to the wave queue.
+template <int DT, int RB>
+__device__ __forceinline__ void Epilogue(const ScoreArgs& A, const float* q_scale_lds, const float* floor_lds,
+                                const uint32_t* mask_lds, uint32_t q0, uint64_t blk, const floatx16 (&acc)[RB],
+                                WaveQueue& Qu) {
+    const uint64_t doc = blk * 32 + (threadIdx.x & 31);
+    const bool doc_ok = doc < A.n_docs;
+    float dscale = 1.0f;
+    if constexpr (DT == FRK_DTYPE_FP8E4M3) dscale = doc_ok ? A.doc_scale[doc] : 0.0f;
+    EpilogueAll<DT, RB>(A, q_scale_lds, floor_lds, mask_lds, q0, doc, doc_ok, dscale, acc, Qu,
+                        std::make_integer_sequence<int, RB>{});
+}
+
+// Probe epilogue: per (query, block) the largest score among the block's docs that pass the filter
+// (-inf if none), written to blockmax[q * A.cap + block index] -- no atomics, no compaction.
+template <int DT, int RB>
+__device__ __forceinline__ void EpilogueBlockMax(const ScoreArgs& A, const float* q_scale_lds, const uint32_t* mask_lds,
+                                        uint32_t q0, uint64_t blk, uint64_t step_index, const floatx16 (&acc)[RB]) {
+    const int lane = threadIdx.x & 63;
+    const uint32_t j = lane & 31;
+    const uint64_t doc = blk * 32 + j;
+    const bool doc_ok = doc < A.n_docs;
+    float dscale = 1.0f;
+    if constexpr (DT == FRK_DTYPE_FP8E4M3) dscale = doc_ok ? A.doc_scale[doc] : 0.0f;
+    const uint32_t half = lane >> 5;
+    float* out = reinterpret_cast<float*>(A.cand);
+#pragma unroll
+    for (int rb = 0; rb < RB; rb++) {
+#pragma unroll
+        for (int g = 0; g < 4; g++) {
+            const int row = 32 * rb + 8 * g + 4 * half;
+            const u32x4 m4 = *reinterpret_cast<const u32x4*>(mask_lds + row);
+#pragma unroll
+            for (int x = 0; x < 4; x++) {
+                float v = acc[rb][4 * g + x];
+                if constexpr (DT == FRK_DTYPE_FP8E4M3) v *= dscale * q_scale_lds[row + x];
+                v = (doc_ok && ((m4[x] >> j) & 1u)) ? v : -INFINITY;
+                for (int off = 16; off > 0; off >>= 1) v = fmaxf(v, __shfl_xor(v, off, 64));
+                const uint32_t q = q0 + row + x;
+                if (j == 0 && q < A.n_queries) out[static_cast<uint64_t>(q) * A.cap + step_index] = v;
+            }
+        }
+    }
+}
+
+template <int DT, int CH, int RB, bool kBlockMax>
+__global__ __launch_bounds__(kThreads, FRK_SCORE_MIN_BLOCKS) void ScoreKernel(ScoreArgs A) {
+    constexpr int G = GroupShape<CH>::G;
+    constexpr int NG = GroupShape<CH>::kGroups;
+    extern __shared__ u32x4 smem[];
+    u32x4* a_lds = smem;                                               // [RB][CH][64]
+    float* q_scale_lds = reinterpret_cast<float*>(smem + RB * CH * 64);  // [32 * RB]
+    float* floor_lds = q_scale_lds + 32 * RB;                          // [32 * RB]
+    uint32_t* masks_lds = reinterpret_cast<uint32_t*>(floor_lds + 32 * RB);  // [waves][32 * RB]
+    uint64_t* queue_lds = reinterpret_cast<uint64_t*>(masks_lds + kWaves * 32 * RB);  // [waves][keys + runs]
+    const uint32_t q0 = blockIdx.y * (32 * RB);
+    StageQueries<DT, CH, RB>(A, a_lds, q_scale_lds, q0);
+    for (int r = threadIdx.x; r < 32 * RB; r += kThreads) {
+        const uint32_t q = q0 + r;
+        // rows past the last query never pass
+        floor_lds[r] = q < A.n_queries ? (A.floor ? A.floor[q] : -INFINITY) : INFINITY;
+    }
+    __syncthreads();
+
+    const int lane = threadIdx.x & 63;
+    const int wave = threadIdx.x >> 6;
+    uint32_t* mask_lds = masks_lds + wave * 32 * RB;
+    WaveQueue Qu;
+    Qu.keys = queue_lds + wave * (kQueueKeys + kQueueRuns);
+    Qu.runs = Qu.keys + kQueueKeys;
+    Qu.nk = Qu.nr = 0;
+    const uint64_t per_wg = (A.block_count + gridDim.x - 1) / gridDim.x;
+    const uint64_t i_lo = static_cast<uint64_t>(blockIdx.x) * per_wg;
+    const uint64_t i_hi = i_lo + per_wg < A.block_count ? i_lo + per_wg : A.block_count;
+    if (i_lo + wave >= i_hi) return;
+    const uint64_t n_mine = (i_hi - i_lo - wave + kWaves - 1) / kWaves;  // blocks of this wave
+    const uint64_t total = n_mine * NG;                                   // (block, group) steps
+
+    auto src = [&](uint64_t step) -> const u32x4* {
+        const uint64_t jj = step / NG;
+        const uint64_t blk = A.block_begin + (i_lo + wave + jj * kWaves) * A.block_step;
+        return A.tiles + (blk * CH + (step % NG) * G) * 64 + lane;
+    };
+    u32x4 ring[kRing][G];
+#pragma unroll
+    for (int s = 0; s < kRing - 1; s++)
+        if (static_cast<uint64_t>(s) < total) {
+            const u32x4* p = src(s);
+#pragma unroll
+            for (int c = 0; c < G; c++) ring[s][c] = p[c * 64];
+        }
+    floatx16 acc[RB];
+#pragma unroll
+    for (int rb = 0; rb < RB; rb++)
+        for (int r = 0; r < 16; r++) acc[rb][r] = 0.0f;
+    // one pipeline step per ring slot; the slot index is a compile-time constant in each call, so the
+    // ring stays in registers without relying on loop unrolling
+    auto step_fn = [&](auto slot_c, uint64_t step) __attribute__((always_inline)) {
+        constexpr int S = decltype(slot_c)::value;
+        const uint64_t ahead = step + kRing - 1;
+        if (ahead < total) {
+            const u32x4* p = src(ahead);
+#pragma unroll
+            for (int c = 0; c < G; c++) ring[(S + kRing - 1) % kRing][c] = p[c * 64];
+        }
+        const int g = static_cast<int>(step % NG);
+        ComputeGroup<DT, CH, RB>(a_lds, g, ring[S], acc);
+        if (g == NG - 1) {
+            const uint64_t i = i_lo + wave + (step / NG) * kWaves;
+            const uint64_t blk = A.block_begin + i * A.block_step;
+            StageMasks<RB>(A, q0, blk, mask_lds);
+            if constexpr (kBlockMax)
+                EpilogueBlockMax<DT, RB>(A, q_scale_lds, mask_lds, q0, blk, i, acc);
+            else
+                Epilogue<DT, RB>(A, q_scale_lds, floor_lds, mask_lds, q0, blk, acc, Qu);
+#pragma unroll
+            for (int rb = 0; rb < RB; rb++)
+#pragma unroll
+                for (int r = 0; r < 16; r++) acc[rb][r] = 0.0f;
+        }
+    };
+    static_assert(kRing == 3, "the step sequence below is written out for a ring of 3");
+    uint64_t step = 0;
+    for (; step + 3 <= total; step += 3) {
+        step_fn(std::integral_constant<int, 0>{}, step);
+        step_fn(std::integral_constant<int, 1>{}, step + 1);
+        step_fn(std::integral_constant<int, 2>{}, step + 2);
+    }
+    if (step < total) step_fn(std::integral_constant<int, 0>{}, step);
+    if (step + 1 < total) step_fn(std::integral_constant<int, 1>{}, step + 1);
+    if constexpr (!kBlockMax) FlushQueue(A, Qu);
+}
+
+using KernelFn = void (*)(ScoreArgs);
+
+template <int DT, int CH, int RB, bool kBlockMax>
+int Launch(const ScoreArgs& A, hipStream_t stream, uint32_t grid_x) {
+    const size_t lds = static_cast<size_t>(RB) * CH * 64 * sizeof(u32x4) + 2 * 32 * RB * sizeof(float) +
+                       kWaves * 32 * RB * sizeof(uint32_t) + kWaves * (kQueueKeys + kQueueRuns) * sizeof(uint64_t);
+    static bool attr_set = false;
+    if (!attr_set) {
+        FRK_HIP_CHECK(hipFuncSetAttribute(reinterpret_cast<const void*>(&ScoreKernel<DT, CH, RB, kBlockMax>),
+                                          hipFuncAttributeMaxDynamicSharedMemorySize, static_cast<int>(lds)));
+        attr_set = true;
+    }
+    const uint32_t groups = (A.n_queries + 32 * RB - 1) / (32 * RB);
+    hipLaunchKernelGGL((ScoreKernel<DT, CH, RB, kBlockMax>), dim3(grid_x, groups), dim3(kThreads), lds, stream, A);
+    FRK_HIP_CHECK(hipGetLastError());
+    return FRK_OK;
+}
+
+// Instantiated shapes: CH = 16-byte chunks per doc. RB = 32-query row blocks per workgroup, chosen so
+// the query tile fits the 160 KiB LDS.
+template <int DT, bool kBlockMax>
+int Dispatch(const ScoreArgs& A, uint32_t ch, hipStream_t stream, uint32_t grid_x) {
+    const uint32_t q = A.n_queries;
+    const int rb = q > 64 ? 4 : (q > 32 ? 2 : 1);
+#define FRK_CASE(CHV)                                                              \
+    if (ch == CHV) {                                                               \
+        if (rb == 4 && CHV * 4 <= 128) return Launch<DT, CHV, 4, kBlockMax>(A, stream, grid_x); \
+        if (rb >= 2 && CHV * 2 <= 128) return Launch<DT, CHV, 2, kBlockMax>(A, stream, grid_x); \
+        return Launch<DT, CHV, 1, kBlockMax>(A, stream, grid_x);                              \
+    }
+    FRK_CASE(4)
+    FRK_CASE(8)
+    FRK_CASE(16)
+    FRK_CASE(32)
+    FRK_CASE(48)
+    FRK_CASE(64)
+#undef FRK_CASE
+    SetError("frk_score_select: %u chunks per doc is not an instantiated shape", ch);
+    return FRK_ERR_UNSUPPORTED;
+}
+
+// Per-dtype entry points, each in its own translation unit (frk_score_<dtype>.hip).
+struct ScoreLaunch {
+    ScoreArgs args;
+    uint32_t chunks;
+    uint32_t grid_x;
+    hipStream_t stream;
+};
+int ScoreF32(const ScoreLaunch& L, bool block_max);
+int ScoreBf16(const ScoreLaunch& L, bool block_max);
+int ScoreFp8(const ScoreLaunch& L, bool block_max);
+}  // namespace frk

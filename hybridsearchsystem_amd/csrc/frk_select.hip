@@ -108,8 +108,64 @@ __global__ __launch_bounds__(kThreads) void SelectKernel(const uint64_t* __restr
     }
 }
 
+// k-th largest of n floats per query (radix select over order-preserving u32 keys, 4 passes).
+__global__ __launch_bounds__(kThreads) void KthLargestKernel(const float* __restrict__ vals, uint32_t stride,
+                                                             uint32_t n, const uint32_t* __restrict__ ks,
+                                                             uint32_t k_default, float* __restrict__ out) {
+    __shared__ uint32_t hist[256];
+    __shared__ uint32_t s_prefix, s_remaining;
+    const uint32_t q = blockIdx.x;
+    const uint32_t k = ks ? ks[q] : k_default;
+    const float* v = vals + static_cast<uint64_t>(q) * stride;
+    if (k == 0 || k > n) {
+        if (threadIdx.x == 0) out[q] = -INFINITY;
+        return;
+    }
+    if (threadIdx.x == 0) {
+        s_prefix = 0;
+        s_remaining = k;
+    }
+    uint32_t mask = 0;
+    for (int shift = 24; shift >= 0; shift -= 8) {
+        for (int b = threadIdx.x; b < 256; b += kThreads) hist[b] = 0;
+        __syncthreads();
+        const uint32_t prefix = s_prefix;
+        for (uint32_t i = threadIdx.x; i < n; i += kThreads) {
+            const uint32_t key = OrderedFromFloat(v[i]);
+            if ((key & mask) == prefix) atomicAdd(&hist[(key >> shift) & 255u], 1u);
+        }
+        __syncthreads();
+        if (threadIdx.x == 0) {
+            uint32_t rem = s_remaining, cum = 0;
+            int d = 255;
+            for (; d > 0; d--) {
+                if (cum + hist[d] >= rem) break;
+                cum += hist[d];
+            }
+            s_remaining = rem - cum;
+            s_prefix = prefix | (static_cast<uint32_t>(d) << shift);
+        }
+        mask |= 0xFFu << shift;
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) out[q] = FloatFromOrdered(s_prefix);
+}
+
 }  // namespace
 }  // namespace frk
+
+extern "C" int frk_kth_largest(const float* d_vals, uint32_t stride, uint32_t n, uint32_t n_queries,
+                               const uint32_t* d_k, uint32_t k_default, float* d_out, frk_stream_t stream) {
+    if (n_queries == 0) return FRK_OK;
+    if (!d_vals || !d_out || stride < n) {
+        frk::SetError("frk_kth_largest: bad arguments");
+        return FRK_ERR_INVALID;
+    }
+    hipLaunchKernelGGL(frk::KthLargestKernel, dim3(n_queries), dim3(frk::kThreads), 0, static_cast<hipStream_t>(stream),
+                       d_vals, stride, n, d_k, k_default, d_out);
+    FRK_HIP_CHECK(hipGetLastError());
+    return FRK_OK;
+}
 
 extern "C" int frk_select_topk(const uint64_t* d_cand, uint32_t cand_cap, const uint32_t* d_counts,
                                uint32_t n_queries, const uint32_t* d_k, uint32_t k_default, uint64_t* d_out,

@@ -1,0 +1,85 @@
+// frk_tile.hip -- device-side tiling of row-major FP32 vectors into a doc store (frk_capi.h, "Doc
+// store"), with BF16 / FP8 E4M3 (+ per-doc scale) conversion. The index loader and builder stream
+// rows through it in chunks, so the FP32 originals never need a second resident copy.
+#include "frk_common.h"
+
+namespace frk {
+namespace {
+
+__global__ __launch_bounds__(256) void RowScaleKernel(const float* __restrict__ rows, uint64_t n, uint32_t dim,
+                                                      float* __restrict__ scales) {
+    const uint64_t r = static_cast<uint64_t>(blockIdx.x) * 4 + (threadIdx.x >> 6);
+    const int lane = threadIdx.x & 63;
+    if (r >= n) return;
+    float mx = 0.0f;
+    for (uint32_t d = lane; d < dim; d += 64) mx = fmaxf(mx, fabsf(rows[r * dim + d]));
+    for (int off = 32; off > 0; off >>= 1) mx = fmaxf(mx, __shfl_xor(mx, off, 64));
+    if (lane == 0) scales[r] = mx > 0.0f ? mx / 448.0f : 1.0f;
+}
+
+__global__ __launch_bounds__(256) void TileKernel(const float* __restrict__ rows, uint64_t n, uint32_t dim,
+                                                  uint64_t first, uint32_t dtype, uint32_t chunks,
+                                                  uint8_t* __restrict__ tiles, const float* __restrict__ scales,
+                                                  uint64_t n_units) {
+    const uint64_t u = static_cast<uint64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (u >= n_units) return;
+    const uint32_t E = dtype == FRK_DTYPE_F32 ? 4 : (dtype == FRK_DTYPE_BF16 ? 8 : 16);
+    const uint64_t b = first / 32 + u / (static_cast<uint64_t>(chunks) * 64);
+    const uint32_t c = static_cast<uint32_t>((u / 64) % chunks);
+    const uint32_t l = static_cast<uint32_t>(u % 64);
+    const uint64_t doc = b * 32 + (l & 31);
+    const uint32_t d0 = c * 2 * E + (l >> 5) * E;
+    float v[16];
+    const bool have = doc >= first && doc < first + n;
+    const float* row = rows + (doc - first) * dim;
+    for (uint32_t e = 0; e < E; e++) v[e] = (have && d0 + e < dim) ? row[d0 + e] : 0.0f;
+    uint32_t w[4];
+    if (dtype == FRK_DTYPE_F32) {
+        for (int e = 0; e < 4; e++) w[e] = __builtin_bit_cast(uint32_t, v[e]);
+    } else if (dtype == FRK_DTYPE_BF16) {
+        for (int e = 0; e < 4; e++)
+            w[e] = static_cast<uint32_t>(F32ToBf16(v[2 * e])) | (static_cast<uint32_t>(F32ToBf16(v[2 * e + 1])) << 16);
+    } else {
+        const float inv = have ? 1.0f / scales[doc - first] : 0.0f;
+        for (int e = 0; e < 4; e++) {
+            uint32_t x = 0;
+            for (int k = 0; k < 4; k++) x |= static_cast<uint32_t>(F32ToE4M3(v[4 * e + k] * inv)) << (8 * k);
+            w[e] = x;
+        }
+    }
+    uint4 out = {w[0], w[1], w[2], w[3]};
+    reinterpret_cast<uint4*>(tiles)[(b * chunks + c) * 64 + l] = out;
+}
+
+}  // namespace
+}  // namespace frk
+
+extern "C" int frk_tile_rows_device(const float* d_rows, uint64_t n_rows, uint32_t dim, uint64_t first_doc,
+                                    uint32_t dtype, uint32_t dim_pad, void* d_tiles, float* d_scales,
+                                    frk_stream_t stream) {
+    const uint32_t cd = frk_chunk_dims(dtype);
+    if (!d_rows || !d_tiles || cd == 0 || dim == 0 || dim_pad < dim || dim_pad % cd != 0 || first_doc % 32 != 0) {
+        frk::SetError("frk_tile_rows_device: bad arguments (dim %u, dim_pad %u, dtype %u, first %llu)", dim, dim_pad,
+                      dtype, (unsigned long long)first_doc);
+        return FRK_ERR_INVALID;
+    }
+    if (n_rows == 0) return FRK_OK;
+    hipStream_t s = static_cast<hipStream_t>(stream);
+    if (dtype == FRK_DTYPE_FP8E4M3) {
+        if (!d_scales) {
+            frk::SetError("frk_tile_rows_device: FP8 needs a scale array");
+            return FRK_ERR_INVALID;
+        }
+        hipLaunchKernelGGL(frk::RowScaleKernel, dim3((n_rows + 3) / 4), dim3(256), 0, s, d_rows, n_rows, dim,
+                           d_scales + first_doc);
+        FRK_HIP_CHECK(hipGetLastError());
+    }
+    const uint32_t chunks = dim_pad / cd;
+    const uint64_t n_blocks = (n_rows + 31) / 32;
+    const uint64_t n_units = n_blocks * chunks * 64;
+    hipLaunchKernelGGL(frk::TileKernel, dim3((n_units + 255) / 256), dim3(256), 0, s, d_rows, n_rows, dim, first_doc,
+                       dtype, chunks, static_cast<uint8_t*>(d_tiles),
+                       dtype == FRK_DTYPE_FP8E4M3 ? d_scales + first_doc : nullptr, n_units);
+    FRK_HIP_CHECK(hipGetLastError());
+    return FRK_OK;
+}

@@ -56,9 +56,25 @@ def filter_eval(d_queries, n_queries, d_expr, d_table, n_segments, seg_docs, n_d
           "frk_filter_eval")
 
 
-def score_select(store, d_queries, n_queries, d_masks, mask_stride, d_floor, block_begin, block_step, block_count,
-                 d_cand, cand_cap, d_counts, stream=None):
-    check(lib().frk_score_select(ctypes.byref(store), ptr(d_queries), n_queries, ptr(d_masks), mask_stride,
+def masks_to_blocks(d_masks, mask_stride, n_queries, n_docs, d_block_masks, q_stride, stream=None):
+    check(lib().frk_masks_to_blocks(ptr(d_masks), mask_stride, n_queries, n_docs, ptr(d_block_masks), q_stride,
+                                    stream_ptr(stream)), "frk_masks_to_blocks")
+
+
+def block_masks_host(bits):
+    """bool [n_queries, n_docs] -> block-major uint32 [n_blocks, n_queries] (bit j = doc 32 b + j)."""
+    bits = np.asarray(bits, dtype=bool)
+    nq, n = bits.shape
+    nb = (n + 31) // 32
+    pad = np.zeros((nq, nb * 32), dtype=bool)
+    pad[:, :n] = bits
+    w = (pad.reshape(nq, nb, 32).astype(np.uint64) << np.arange(32, dtype=np.uint64)).sum(axis=2, dtype=np.uint64)
+    return np.ascontiguousarray(w.T.astype(np.uint32))
+
+
+def score_select(store, d_queries, n_queries, d_block_masks, mask_q_stride, d_floor, block_begin, block_step,
+                 block_count, d_cand, cand_cap, d_counts, stream=None):
+    check(lib().frk_score_select(ctypes.byref(store), ptr(d_queries), n_queries, ptr(d_block_masks), mask_q_stride,
                                  ptr(d_floor), block_begin, block_step, block_count, ptr(d_cand), cand_cap,
                                  ptr(d_counts), stream_ptr(stream)), "frk_score_select")
 

@@ -1,0 +1,132 @@
+"""Index tables and batch search through the C ABI handle layer (include/frk_capi.h, layer 2).
+
+``Table.build`` stages an index in device memory from arrays (vectors host- or device-resident,
+per-tag posting lists as CSR); ``Table.load`` reads an index directory. ``Table.search`` runs one
+batch through ``NpuRetrieval::FullRecallSearcher``.
+"""
+import ctypes
+
+import numpy as np
+
+from . import check, lib
+from ._capi_sigs import SearchOptions, SearchStats
+
+FRK_DTYPE_F32, FRK_DTYPE_BF16, FRK_DTYPE_FP8E4M3 = 0, 1, 2
+
+
+def _mask(dtypes):
+    m = 0
+    for d in dtypes:
+        m |= 1 << int(d)
+    return m
+
+
+class Table:
+    def __init__(self, handle, vector_field="vec", posting_field="tag", dim=None):
+        self.h = handle
+        self.vector_field = vector_field
+        self.posting_field = posting_field
+        self.dim = dim
+        n = ctypes.c_uint64()
+        s = ctypes.c_uint32()
+        d = ctypes.c_uint32()
+        check(lib().frk_table_info(self.h, ctypes.byref(n), ctypes.byref(s), ctypes.byref(d)), "frk_table_info")
+        self.n_docs, self.n_segments, self.docs_per_segment = int(n.value), int(s.value), int(d.value)
+
+    def __del__(self):
+        self.free()
+
+    def free(self):
+        if getattr(self, "h", None):
+            lib().frk_table_free(self.h)
+            self.h = None
+
+    @staticmethod
+    def load(path, device=0, dtypes=(), vector_field="vec", posting_field="tag", dim=None):
+        h = ctypes.c_void_p()
+        check(lib().frk_table_load(device, path.encode(), _mask(dtypes), ctypes.byref(h)), "frk_table_load")
+        return Table(h, vector_field, posting_field, dim)
+
+    @staticmethod
+    def build(vectors, tag_offsets, tag_doc_ids, n_docs=None, tags=None, docs_per_segment=131072, dtypes=(),
+              vector_field="vec", posting_field="tag", is_matrix=False, threshold=0.05, packed=True, gdocids=None,
+              device=0, chunk_rows=1 << 20):
+        """vectors: numpy [n, dim] float32 or a torch CUDA tensor; tag postings as CSR over tags
+        0..T-1 (or the ids in `tags`): docs of tag t are tag_doc_ids[tag_offsets[t]:tag_offsets[t+1]]."""
+        L = lib()
+        n = int(n_docs if n_docs is not None else vectors.shape[0])
+        dim = int(vectors.shape[1])
+        b = ctypes.c_void_p()
+        check(L.frk_builder_create(device, n, docs_per_segment, ctypes.byref(b)), "frk_builder_create")
+        try:
+            check(L.frk_builder_add_vector_field(b, vector_field.encode(), dim, _mask(dtypes)), "add_vector_field")
+            on_dev = hasattr(vectors, "is_cuda") and vectors.is_cuda
+            if on_dev:
+                vt = vectors.contiguous()
+                for at in range(0, n, chunk_rows):
+                    m = min(chunk_rows, n - at)
+                    check(L.frk_builder_set_vectors(b, vector_field.encode(), vt[at:at + m].data_ptr(), at, m, 1),
+                          "set_vectors")
+            else:
+                v = np.ascontiguousarray(vectors, dtype=np.float32)
+                check(L.frk_builder_set_vectors(b, vector_field.encode(), v.ctypes.data, 0, n, 0), "set_vectors")
+            check(L.frk_builder_add_posting_field(b, posting_field.encode(), int(is_matrix), float(threshold),
+                                                  int(packed)), "add_posting_field")
+            off = np.ascontiguousarray(tag_offsets, dtype=np.uint64)
+            ids = np.ascontiguousarray(tag_doc_ids, dtype=np.uint32)
+            t = np.ascontiguousarray(tags if tags is not None else np.arange(len(off) - 1), dtype=np.uint64)
+            check(L.frk_builder_add_tag_postings(b, posting_field.encode(), len(t), t.ctypes.data, off.ctypes.data,
+                                                 ids.ctypes.data if len(ids) else None), "add_tag_postings")
+            if gdocids is not None:
+                g = np.ascontiguousarray(gdocids, dtype=np.uint64)
+                check(L.frk_builder_set_gdocids(b, g.ctypes.data, len(g)), "set_gdocids")
+            h = ctypes.c_void_p()
+            check(L.frk_builder_finish(b, ctypes.byref(h)), "frk_builder_finish")
+        finally:
+            L.frk_builder_free(b)
+        return Table(h, vector_field, posting_field, dim)
+
+    def compile_filter(self, expr):
+        cap = 4 * len(expr) + 16
+        buf = np.zeros(cap, dtype=np.uint32)
+        npost, ops, ma = ctypes.c_uint32(), ctypes.c_uint32(), ctypes.c_int32()
+        n = lib().frk_compile_filter(self.h, self.posting_field.encode(), expr.encode(), buf.ctypes.data, cap,
+                                     ctypes.byref(npost), ctypes.byref(ops), ctypes.byref(ma))
+        check(0 if n >= 0 else n, "frk_compile_filter")
+        return buf[:n].copy(), int(npost.value), int(ops.value), bool(ma.value)
+
+    def search(self, queries, filters, ks, dtype=FRK_DTYPE_F32, rescore_factor=4, probe=True, probe_step=0,
+               floors=None, cand_cap=0, k_stride=None):
+        """Returns (ids int64 [Q, k_stride], scores float32, counts uint32, passing uint32, stats dict)."""
+        q = np.ascontiguousarray(queries, dtype=np.float32)
+        nq, dim = q.shape
+        ks = np.ascontiguousarray(np.broadcast_to(np.asarray(ks, dtype=np.uint32), (nq,)))
+        k_stride = int(k_stride or max(1, int(ks.max()) if nq else 1))
+        fl = None
+        if filters is not None:
+            enc = [f.encode() if f is not None else None for f in filters]
+            fl = (ctypes.c_char_p * nq)(*enc)
+        opts = SearchOptions(dtype, rescore_factor, int(bool(probe)), probe_step, None, cand_cap)
+        if floors is not None:
+            fa = np.ascontiguousarray(floors, dtype=np.float32)
+            opts.floors = fa.ctypes.data
+        ids = np.full((nq, k_stride), -1, dtype=np.int64)
+        sc = np.zeros((nq, k_stride), dtype=np.float32)
+        cnt = np.zeros(nq, dtype=np.uint32)
+        passing = np.zeros(nq, dtype=np.uint32)
+        st = SearchStats()
+        check(lib().frk_search(self.h, self.vector_field.encode(), self.posting_field.encode(), nq, q.ctypes.data,
+                               dim, fl, ks.ctypes.data, ctypes.byref(opts), k_stride, ids.ctypes.data, sc.ctypes.data,
+                               cnt.ctypes.data, passing.ctypes.data, ctypes.byref(st)), "frk_search")
+        stats = {f: getattr(st, f) for f, _ in SearchStats._fields_}
+        return ids, sc, cnt, passing, stats
+
+
+def parse_filter(expr):
+    """The product's filter grammar as (n, 2) int64 pre-order pairs; None if it does not parse."""
+    cap = 4 * len(expr) + 8
+    buf = np.zeros(2 * cap, dtype=np.int64)
+    n = lib().frk_parse_filter(expr.encode(), buf.ctypes.data, cap)
+    if n < 0:
+        return None
+    return buf[:2 * n].reshape(n, 2)

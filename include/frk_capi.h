@@ -104,23 +104,42 @@ uint64_t frk_tiles_bytes(uint64_t n_docs, uint32_t dim_pad, uint32_t dtype);
 int frk_tile_docs(const float* src, uint64_t n_docs, uint32_t dim, uint32_t dtype, uint32_t dim_pad, void* dst_tiles,
                   float* dst_scales, uint32_t n_threads);
 
+/* Device-side staging: rows [first_doc, first_doc + n_rows) of a store from row-major FP32 device
+ * rows; first_doc must be a multiple of 32, and docs of the last touched block past n_rows are
+ * zero-filled. FP8 writes d_scales[first_doc + r]. Stream-ordered. */
+int frk_tile_rows_device(const float* d_rows, uint64_t n_rows, uint32_t dim, uint64_t first_doc, uint32_t dtype,
+                         uint32_t dim_pad, void* d_tiles, float* d_scales, frk_stream_t stream);
+
+/* Filter output (query-major u64 words, as frk_filter_eval writes it) -> the block-major 32-bit masks
+ * the scorer reads: out[b * q_stride + q] = docs 32b .. 32b+31 of query q (0 for q >= n_queries). */
+int frk_masks_to_blocks(const uint64_t* d_masks, uint64_t mask_stride, uint32_t n_queries, uint64_t n_docs,
+                        uint32_t* d_block_masks, uint32_t q_stride, frk_stream_t stream);
+
 /* ------------------------------------------------------------------------------------------------
  * Fused scorer: S[q][n] = <query q, doc n> on MFMA (FP32 accumulation), with the filter mask and a
  * per-query score floor fused in, and survivors compacted by wave ballot + prefix count into
  * per-query candidate lists. Replaces kernel_vector_mmad (scorer/kernel_vector_mmad_op.h:21-199)
  * and Aggregator (aggregator/kernel_aggregator.h:22-182): no score matrix is written.
  *
- * Doc blocks visited: block_begin + i * block_step for i < block_count (the probe pass samples
- * with a step > 1). A doc n passes if mask bit n of query q is set (d_masks row q at stride
- * mask_stride words, bit n%64 of word n/64; NULL = all docs) and S[q][n] >= floor[q] (d_floor NULL
- * = no floor). A passing doc appends MakeKey(S, n) (score-ordered high word, reversed doc id low
- * word) to cand[q * cand_cap + slot]; counts[q] counts every pass, so counts[q] > cand_cap means
- * the list overflowed and was truncated. d_queries: [n_queries][dim] FP32 row-major.
+ * Doc blocks visited: block_begin + i * block_step for i < block_count. A doc n passes if its bit in
+ * the block-major filter masks is set (d_block_masks[(n / 32) * mask_q_stride + q], bit n % 32;
+ * NULL = every doc; see frk_masks_to_blocks) and S[q][n] >= floor[q] (d_floor NULL = no floor). A
+ * passing doc appends MakeKey(S, n) (score-ordered high word, reversed doc id low word) to
+ * cand[q * cand_cap + slot]; counts[q] counts every pass, so counts[q] > cand_cap means the list
+ * overflowed and was truncated. d_queries: [n_queries][dim] FP32 row-major.
  * ------------------------------------------------------------------------------------------------ */
 int frk_score_select(const frk_doc_store* store, const float* d_queries, uint32_t n_queries,
-                     const uint64_t* d_masks, uint64_t mask_stride, const float* d_floor,
+                     const uint32_t* d_block_masks, uint32_t mask_q_stride, const float* d_floor,
                      uint64_t block_begin, uint64_t block_step, uint64_t block_count, uint64_t* d_cand,
                      uint32_t cand_cap, uint32_t* d_counts, frk_stream_t stream);
+
+/* Probe pass of the same kernel: per (query q, sampled block i) the largest score among the block's
+ * filter-passing docs, or -inf, written to d_blockmax[q * stride + i] (no candidate lists, no
+ * atomics). The k-th largest of a query's block maxima is a valid score floor: the k blocks that
+ * reach it each hold a distinct passing doc at least that good. */
+int frk_score_blockmax(const frk_doc_store* store, const float* d_queries, uint32_t n_queries,
+                       const uint32_t* d_block_masks, uint32_t mask_q_stride, uint64_t block_begin, uint64_t block_step,
+                       uint64_t block_count, float* d_blockmax, uint32_t stride, frk_stream_t stream);
 
 /* ------------------------------------------------------------------------------------------------
  * Exact top-k selection over candidate keys: radix select of the k-th largest key, then the k keys
@@ -132,9 +151,93 @@ int frk_score_select(const frk_doc_store* store, const float* d_queries, uint32_
  * ------------------------------------------------------------------------------------------------ */
 #define FRK_SELECT_MAX_K 4096
 
+/* k-th largest of the n floats vals[q * stride ..] per query (-inf when k > n or k == 0); used on
+ * block maxima to turn the probe into a score floor. */
+int frk_kth_largest(const float* d_vals, uint32_t stride, uint32_t n, uint32_t n_queries, const uint32_t* d_k,
+                    uint32_t k_default, float* d_out, frk_stream_t stream);
+
 int frk_select_topk(const uint64_t* d_cand, uint32_t cand_cap, const uint32_t* d_counts, uint32_t n_queries,
                     const uint32_t* d_k, uint32_t k_default, uint64_t* d_out, uint32_t k_stride,
                     uint32_t* d_out_counts, float* d_kth_score, frk_stream_t stream);
+
+/* ------------------------------------------------------------------------------------------------
+ * FP32 rescoring of candidates after a low-precision pass: keys_out[q][i] = MakeKey(<q, doc>, doc)
+ * for doc = the doc of keys_in[q][i], i < min(counts[q], stride), computed from the FP32 store.
+ * d_queries: [n_queries][f32_store->dim] FP32. No reference counterpart (the reference scores FP16
+ * only, vector_scorer_mmad.cpp:75-211).
+ * ------------------------------------------------------------------------------------------------ */
+int frk_rescore(const frk_doc_store* f32_store, const float* d_queries, uint32_t n_queries, const uint64_t* d_keys_in,
+                uint32_t stride, const uint32_t* d_counts, uint64_t* d_keys_out, frk_stream_t stream);
+
+/* ================================================================================================
+ * Layer 2: index and searcher handles (C face of NpuRetrieval::DataTable / FullRecallSearcher).
+ * ================================================================================================ */
+typedef struct frk_table frk_table;
+typedef struct frk_builder frk_builder;
+
+/* DataTable::LoadData (engine/index/data_table.h:20): the index directory written by the builder
+ * (.meta, id/ *.idm files, per-field .vector and .posting files). dtype_mask: extra low-precision stores to
+ * build, bits (1 << FRK_DTYPE_BF16) | (1 << FRK_DTYPE_FP8E4M3); the FP32 store is always built. */
+int frk_table_load(int32_t device, const char* dir, uint32_t dtype_mask, frk_table** out);
+void frk_table_free(frk_table* t);
+int frk_table_info(const frk_table* t, uint64_t* n_docs, uint32_t* n_segments, uint32_t* docs_per_segment);
+
+/* In-memory build (the offline builder's output without files). Posting lists per tag are CSR:
+ * tags[t]'s docs are doc_ids[offsets[t] .. offsets[t+1]), strictly ascending; the token is
+ * Hash64(decimal tag) (harness/query/filter_setup.cpp:100). Vectors: row-major FP32 rows
+ * [first, first + n), host or device memory, first a multiple of 32. */
+int frk_builder_create(int32_t device, uint32_t n_docs, uint32_t docs_per_segment, frk_builder** out);
+void frk_builder_free(frk_builder* b);
+int frk_builder_add_vector_field(frk_builder* b, const char* name, uint32_t dim, uint32_t dtype_mask);
+int frk_builder_set_vectors(frk_builder* b, const char* name, const float* rows, uint64_t first, uint64_t n,
+                            int rows_on_device);
+int frk_builder_add_posting_field(frk_builder* b, const char* name, int is_matrix, float density_threshold,
+                                  int packed);
+int frk_builder_add_tag_postings(frk_builder* b, const char* field, uint64_t n_tags, const uint64_t* tags,
+                                 const uint64_t* offsets, const uint32_t* doc_ids);
+int frk_builder_set_gdocids(frk_builder* b, const uint64_t* ids, uint64_t n);
+int frk_builder_finish(frk_builder* b, frk_table** out); /* the builder stays owned by the caller */
+
+/* Host codecs of the index format: the token hash (core/hash/hash.cpp:10-16) and the posting
+ * serializer (format/posting_serializer.cpp:81-211; SPARSE_IDLIST keeps the given id order).
+ * frk_serialize_posting returns the byte count or a negative error. */
+uint64_t frk_hash64(const char* data, uint32_t len);
+int64_t frk_serialize_posting(const uint32_t* ids, uint32_t n_ids, uint32_t doc_num, int is_matrix, float threshold,
+                              int packed, uint8_t* out, uint64_t cap);
+
+/* The search driver's filter grammar (harness/query/filter_expr.h:164-183) as pre-order
+ * (type, arg) pairs: 0 TERM(tag), 1 AND(n children), 2 OR(n), 3 NOT(1). Returns the pair count,
+ * 0 for an empty (match-all) filter, or a negative error. */
+int frk_parse_filter(const char* expr, int64_t* pairs, int32_t cap_pairs);
+
+/* The postfix program a filter compiles to against a table (TextFilter::PrepareExpr,
+ * text_filter.cpp:432-467). Returns the program length or a negative error. */
+int frk_compile_filter(const frk_table* t, const char* field, const char* expr, uint32_t* out_expr, int32_t cap,
+                       uint32_t* n_postings, uint32_t* op_num, int32_t* match_all);
+
+typedef struct {
+    uint32_t dtype;           /* scoring store: FRK_DTYPE_F32 (plain), _BF16 or _FP8E4M3 */
+    uint32_t rescore_factor;  /* low precision: FP32 rescoring of the top k * m */
+    int32_t probe;            /* sampled k-th-best floor (exact pruning) */
+    uint32_t probe_step;      /* 0 = automatic */
+    const float* floors;      /* host, per query, or NULL: caller score floors (learned bound) */
+    uint32_t cand_cap;        /* 0 = automatic */
+} frk_search_options;
+
+typedef struct {
+    float ms_filter, ms_probe, ms_score, ms_select, ms_rescore, ms_total;
+    uint64_t candidates;
+    uint32_t reruns, overflowed, probe_step, cand_cap;
+} frk_search_stats;
+
+/* FullRecallSearcher::AddQuery x n + BatchSearch (full_recall_searcher.h:30-33) with filter strings
+ * (NULL entry or NULL array = match all). Results per query: out_ids[q * k_stride + i] (global doc
+ * ids) and out_scores, best first, out_counts[q] = min(k, passing docs); out_passing[q] = docs that
+ * passed the filter (optional). */
+int frk_search(const frk_table* t, const char* vector_field, const char* posting_field, uint32_t n_queries,
+               const float* queries, uint32_t dim, const char* const* filters, const uint32_t* ks,
+               const frk_search_options* opts, uint32_t k_stride, int64_t* out_ids, float* out_scores,
+               uint32_t* out_counts, uint32_t* out_passing, frk_search_stats* stats);
 
 #ifdef __cplusplus
 }

@@ -1,0 +1,96 @@
+"""CPU checks of the product library: it loads without a GPU, exports every C-ABI symbol the header
+declares, and its host codecs (token hash, posting serializer, filter grammar) reproduce the
+reference's golden vectors."""
+import os
+import re
+
+import numpy as np
+import pytest
+
+import oracle
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+GOLD = os.path.join(ROOT, "tests", "golden", "ref_vectors.npz")
+
+
+@pytest.fixture(scope="module")
+def L():
+    import hybridsearchsystem_amd as h
+    return h.lib()
+
+
+def test_library_exports_every_declared_symbol(L):
+    hdr = open(os.path.join(ROOT, "include", "frk_capi.h")).read()
+    hdr = re.sub(r"/\*.*?\*/", "", hdr, flags=re.S)
+    names = set(re.findall(r"\b(frk_[a-z0-9_]+)\s*\(", hdr))
+    assert len(names) >= 20
+    missing = [n for n in sorted(names) if not hasattr(L, n)]
+    assert not missing, missing
+
+
+def test_hash64_matches_reference_vectors(L):
+    g = np.load(GOLD, allow_pickle=False)
+    for t, v in zip(g["hash_tags"][:2000], g["hash_tag_vals"][:2000]):
+        s = str(int(t)).encode()
+        assert L.frk_hash64(s, len(s)) == int(v)
+    raw = g["hash_raw"].tobytes()
+    off = 0
+    for n, v in enumerate(g["hash_raw_vals"]):
+        assert L.frk_hash64(raw[off:off + n], n) == int(v)
+        off += n
+
+
+def test_posting_serializer_matches_reference_bytes(L):
+    g = np.load(GOLD, allow_pickle=False)
+    ids_off = b_off = 0
+    for (doc_num, is_matrix, packed, _), nid, nb in zip(g["post_cases"], g["post_ids_len"], g["post_bytes_len"]):
+        ids = np.ascontiguousarray(g["post_ids"][ids_off:ids_off + nid], dtype=np.uint32)
+        want = g["post_bytes"][b_off:b_off + nb].tobytes()
+        ids_off += nid
+        b_off += nb
+        cap = 64 + 6 * max(len(ids), 1) + 2 * int(doc_num)
+        out = np.zeros(cap, np.uint8)
+        n = L.frk_serialize_posting(ids.ctypes.data, len(ids), int(doc_num), int(is_matrix), 0.05, int(packed),
+                                    out.ctypes.data, cap)
+        got = out[:n].tobytes()
+        if (want[3] >> 4) == 3:
+            assert got[:8] == want[:8]
+            assert sorted(np.frombuffer(got[8:], np.uint32)) == sorted(np.frombuffer(want[8:], np.uint32))
+        else:
+            assert got == want
+
+
+def test_filter_grammar_matches_reference(L):
+    from hybridsearchsystem_amd.search import parse_filter
+    g = np.load(GOLD, allow_pickle=False)
+    off = 0
+    for e, n in zip(g["filter_exprs"], g["filter_pairs_len"]):
+        got = parse_filter(str(e))
+        if n < 0:
+            assert got is None, e
+            continue
+        assert got is not None and got.shape[0] == n, e
+        if n:
+            assert np.array_equal(got.reshape(-1), g["filter_pairs"][2 * off:2 * (off + n)]), e
+        off += n
+
+
+def test_synthetic_filters_hit_their_selectivity_range():
+    from hybridsearchsystem_amd import synth
+    dens = synth.zipf_densities(1024)
+    fs, sels = synth.random_filters(7, dens, 200, sel=(0.0001, 0.6))
+    assert len(fs) == 200 and sels.min() >= 0.0001 and sels.max() <= 0.6
+    for f in fs[:50]:
+        assert oracle.parse_filter(f) is not None
+
+
+def test_synthetic_postings_follow_densities():
+    from hybridsearchsystem_amd import synth
+    rng = np.random.default_rng(1)
+    dens = np.array([0.5, 0.05, 0.001, 1e-5])
+    off, ids = synth.gen_postings(rng, 200000, dens)
+    counts = np.diff(off.astype(np.int64))
+    assert np.all(np.abs(counts - dens * 200000) <= 6 * np.sqrt(dens * 200000) + 2)
+    for t in range(4):
+        d = ids[off[t]:off[t + 1]]
+        assert np.all(np.diff(d.astype(np.int64)) > 0) and (len(d) == 0 or d[-1] < 200000)

@@ -161,14 +161,24 @@ def _run_topk(frk, docs, qs, masks_words, ks, dtype=0, floor=None, block_step=1,
     nq = qs.shape[0]
     d_q = dev(qp)
     mw = (n + 63) // 64
-    d_m = dev(masks_words.view(np.int64)) if masks_words is not None else None
+    d_m = None
+    if masks_words is not None:
+        # the scorer reads block-major masks: transposed on the device, checked against the host layout
+        qs_stride = (nq + 31) // 32 * 32
+        d_words = dev(masks_words.view(np.int64))
+        d_m = torch.zeros(((n + 31) // 32, qs_stride), dtype=torch.int32, device="cuda")
+        frk.masks_to_blocks(d_words, mw, nq, n, d_m, qs_stride)
+        bits = np.stack([oracle.bits_from_words(w, n) for w in masks_words])
+        want = frk.block_masks_host(bits)
+        assert np.array_equal(d_m.cpu().numpy().view(np.uint32)[:, :nq], want)
     cap = cap or n
     d_cand = torch.zeros((nq, cap), dtype=torch.int64, device="cuda")
     d_cnt = torch.zeros(nq, dtype=torch.int32, device="cuda")
     n_blocks = (n + 31) // 32
     count = (n_blocks + block_step - 1) // block_step
     d_floor = dev(floor.astype(np.float32)) if floor is not None else None
-    frk.score_select(store, d_q, nq, d_m, mw, d_floor, 0, block_step, count, d_cand, cap, d_cnt)
+    frk.score_select(store, d_q, nq, d_m, d_m.shape[1] if d_m is not None else 0, d_floor, 0, block_step, count,
+                     d_cand, cap, d_cnt)
     kmax = int(max(ks))
     d_k = dev(np.asarray(ks, np.int32))
     d_out = torch.zeros((nq, kmax), dtype=torch.int64, device="cuda")
@@ -257,3 +267,46 @@ def test_candidate_overflow_is_reported(frk):
     qs = rng.standard_normal((4, 64)).astype(np.float32)
     keys, oc, cnt, kth = _run_topk(frk, docs, qs, None, np.full(4, 10, np.int32), cap=100)
     assert np.all(cnt == 3000)  # counts keep counting past the capacity; the caller detects cnt > cap
+
+
+def test_blockmax_probe_and_kth_largest(frk):
+    rng = np.random.default_rng(44)
+    n, dim, nq, step = 30000, 128, 40, 3
+    docs, cent = _gen.mixture_corpus(rng, n, dim, 16)
+    qs = _gen.mixture_queries(rng, cent, nq)
+    bits = rng.random((nq, n)) < rng.uniform(0.0005, 0.5, (nq, 1))
+    masks = np.stack([oracle.words_from_bits(b) for b in bits])
+    tiles, scales, dp = frk.tile_docs(docs, 0)
+    t_tiles = dev(tiles)
+    store = frk.doc_store(t_tiles, None, 0, dp, n)
+    nb = (n + 31) // 32
+    cnt = (nb + step - 1) // step
+    d_bm = torch.zeros((nq, cnt), dtype=torch.float32, device="cuda")
+    import ctypes
+    from hybridsearchsystem_amd import check, lib
+    d_qs, d_masks = dev(qs), dev(frk.block_masks_host(bits))  # kept alive until the kernels ran
+    check(lib().frk_score_blockmax(ctypes.byref(store), d_qs.data_ptr(), nq, d_masks.data_ptr(), nq, 0,
+                                   step, cnt, d_bm.data_ptr(), cnt, None), "blockmax")
+    ks = rng.choice([1, 5, 50, 400, 5000], size=nq).astype(np.int32)
+    d_kth = torch.zeros(nq, dtype=torch.float32, device="cuda")
+    d_ks = dev(ks)
+    check(lib().frk_kth_largest(d_bm.data_ptr(), cnt, cnt, nq, d_ks.data_ptr(), 0, d_kth.data_ptr(), None), "kth")
+    torch.cuda.synchronize()
+    bm = d_bm.cpu().numpy()
+    kth = d_kth.cpu().numpy()
+    s64 = docs.astype(np.float64) @ qs.astype(np.float64).T  # [n, nq]
+    for q in range(nq):
+        for i in range(cnt):
+            lo, hi = i * step * 32, min(n, i * step * 32 + 32)
+            sel = np.nonzero(bits[q, lo:hi])[0] + lo
+            if len(sel) == 0:
+                assert bm[q, i] == -np.inf
+            else:
+                want = s64[sel, q].max()
+                assert abs(bm[q, i] - want) <= 1e-5 * np.abs(docs[sel] * qs[q]).sum(axis=1).max()
+        srt = np.sort(bm[q])[::-1]
+        assert kth[q] == (srt[ks[q] - 1] if ks[q] <= cnt else -np.inf)
+        # the floor is a valid lower bound on the k-th best passing score (FP32 GPU scores)
+        finite = (srt > -np.inf).sum()
+        if ks[q] <= finite:
+            assert (s64[bits[q], q] >= kth[q] - 1e-4).sum() >= ks[q]
