@@ -43,7 +43,7 @@ def lib():
     vp, u32, u64, i32 = ctypes.c_void_p, ctypes.c_uint32, ctypes.c_uint64, ctypes.c_int
     L.frk_last_error.restype = ctypes.c_char_p
     L.frk_filter_eval.restype = i32
-    L.frk_filter_eval.argtypes = [vp, u32, vp, vp, u32, u32, u64, vp, u64, vp, vp, vp]
+    L.frk_filter_eval.argtypes = [vp, u32, vp, vp, u32, u32, u64, vp, u64, vp, vp, u32, vp]
     L.frk_chunk_dims.restype = u32
     L.frk_chunk_dims.argtypes = [u32]
     L.frk_score_select.restype = i32

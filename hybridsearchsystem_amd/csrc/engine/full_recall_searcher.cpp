@@ -216,6 +216,7 @@ bool FullRecallSearcher::RunDevice(const VectorFieldData& vf, std::vector<AggrDe
     std::vector<frk_filter_query> descs(Q);
     std::vector<uint32_t> expr;
     std::vector<uint64_t> table;
+    uint32_t maxPostings = 0;
     for (uint32_t q = 0; q < Q; q++) {
         FilterExpr fe;
         if (!PrepareFilterExpr(*m_table, m_trees[q], fe)) return false;
@@ -226,6 +227,7 @@ bool FullRecallSearcher::RunDevice(const VectorFieldData& vf, std::vector<AggrDe
         d.expr_len = fe.emptyTree ? 0 : static_cast<uint32_t>(fe.postExpr.size());
         d.n_postings = fe.emptyTree ? 0 : static_cast<uint32_t>(fe.postingDeviceAddrs.size());
         d.op_num = fe.opNum;
+        maxPostings = std::max(maxPostings, d.n_postings);
         if (fe.emptyTree) continue;
         expr.insert(expr.end(), fe.postExpr.begin(), fe.postExpr.end());
         for (uint32_t s = 0; s < segs; s++)
@@ -328,7 +330,7 @@ bool FullRecallSearcher::RunDevice(const VectorFieldData& vf, std::vector<AggrDe
     uint64_t* dMask = ws->bitmaps.As<uint64_t>();
     if (frk_filter_eval(reinterpret_cast<const frk_filter_query*>(db + oDesc), Q,
                         reinterpret_cast<const uint32_t*>(db + oExpr), reinterpret_cast<const uint64_t*>(db + oTab),
-                        segs, dnps, nDocs, dMask, maskStride, dCnt + cFilter, dCnt + cStatus, s) != 0)
+                        segs, dnps, nDocs, dMask, maskStride, dCnt + cFilter, dCnt + cStatus, maxPostings, s) != 0)
         return fail("filter");
     uint32_t* dBMask = ws->bmasks.As<uint32_t>();
     if (frk_masks_to_blocks(dMask, maskStride, Q, nDocs, dBMask, qStride, s) != 0) return fail("mask transpose");

@@ -325,11 +325,42 @@ __device__ __forceinline__ void Epilogue(const ScoreArgs& A, const float* q_scal
                         std::make_integer_sequence<int, RB>{});
 }
 
+// One stage of the block-max butterfly over lane bit B with N rows left per lane (M = array size):
+// a lane keeps the lower or upper half of its rows (by its bit B) and maxes in the partner's copy of
+// the same rows; once one row is left the remaining stages are plain exchanges. Recursion keeps every
+// index a compile-time constant so the rows stay in registers.
+template <int M, int N, int B>
+__device__ __forceinline__ void BlockMaxStages(float (&v)[M], uint32_t j, uint32_t& base) {
+    if constexpr (B >= 1) {
+        if constexpr (N > 1) {
+            const bool upper = (j & B) != 0;
+#pragma unroll
+            for (int i = 0; i < N / 2; i++) {
+                const float send = upper ? v[i] : v[i + N / 2];
+                const float keep = upper ? v[i + N / 2] : v[i];
+                v[i] = fmaxf(keep, __shfl_xor(send, B, 64));
+            }
+            if (upper) base += N / 2;
+            BlockMaxStages<M, N / 2, B / 2>(v, j, base);
+        } else {
+            v[0] = fmaxf(v[0], __shfl_xor(v[0], B, 64));
+            BlockMaxStages<M, 1, B / 2>(v, j, base);
+        }
+    }
+}
+
 // Probe epilogue: per (query, block) the largest score among the block's docs that pass the filter
 // (-inf if none), written to blockmax[q * A.cap + block index] -- no atomics, no compaction.
+// The max over the 32 docs (lanes j of each half) is a butterfly transpose-reduce: at each of the five
+// lane-bit stages a lane keeps half of its rows and trades the other half with its partner, so the
+// 16 RB rows a lane holds shrink to 16 RB / 32 after 16 RB - 1 exchanges in total (instead of five
+// exchanges per row). After it, lane j of half h holds the rows of flat accumulator indices 2j, 2j+1
+// (RB = 4), or the row of index j (RB = 2); with RB = 1 the last stage is a plain exchange of the
+// single remaining row, index j / 2, which lanes j and j ^ 1 then share.
 template <int DT, int RB>
 __device__ __forceinline__ void EpilogueBlockMax(const ScoreArgs& A, const float* q_scale_lds, const uint32_t* mask_lds,
                                         uint32_t q0, uint64_t blk, uint64_t step_index, const floatx16 (&acc)[RB]) {
+    constexpr int N = 16 * RB;
     const int lane = threadIdx.x & 63;
     const uint32_t j = lane & 31;
     const uint64_t doc = blk * 32 + j;
@@ -337,7 +368,7 @@ __device__ __forceinline__ void EpilogueBlockMax(const ScoreArgs& A, const float
     float dscale = 1.0f;
     if constexpr (DT == FRK_DTYPE_FP8E4M3) dscale = doc_ok ? A.doc_scale[doc] : 0.0f;
     const uint32_t half = lane >> 5;
-    float* out = reinterpret_cast<float*>(A.cand);
+    float v[N];
 #pragma unroll
     for (int rb = 0; rb < RB; rb++) {
 #pragma unroll
@@ -346,14 +377,25 @@ __device__ __forceinline__ void EpilogueBlockMax(const ScoreArgs& A, const float
             const u32x4 m4 = *reinterpret_cast<const u32x4*>(mask_lds + row);
 #pragma unroll
             for (int x = 0; x < 4; x++) {
-                float v = acc[rb][4 * g + x];
-                if constexpr (DT == FRK_DTYPE_FP8E4M3) v *= dscale * q_scale_lds[row + x];
-                v = (doc_ok && ((m4[x] >> j) & 1u)) ? v : -INFINITY;
-                for (int off = 16; off > 0; off >>= 1) v = fmaxf(v, __shfl_xor(v, off, 64));
-                const uint32_t q = q0 + row + x;
-                if (j == 0 && q < A.n_queries) out[static_cast<uint64_t>(q) * A.cap + step_index] = v;
+                float s = acc[rb][4 * g + x];
+                if constexpr (DT == FRK_DTYPE_FP8E4M3) s *= dscale * q_scale_lds[row + x];
+                v[16 * rb + 4 * g + x] = (doc_ok && ((m4[x] >> j) & 1u)) ? s : -INFINITY;
             }
         }
+    }
+    uint32_t base = 0;  // flat index of v[0] after the halving stages
+    BlockMaxStages<N, N, 16>(v, j, base);
+    constexpr int n = N >= 32 ? N / 32 : 1;  // rows left per lane
+    float* out = reinterpret_cast<float*>(A.cand);
+#pragma unroll
+    for (int e = 0; e < n; e++) {
+        // fewer than 32 rows: the last log2(32 / N) stages were plain exchanges, so lanes that differ
+        // only in those low bits hold the same row; the lowest of them writes it
+        if (N < 32 && (j & (32 / N - 1)) != 0) break;
+        const uint32_t r = base + e;
+        const uint32_t row = 32 * (r >> 4) + 8 * ((r >> 2) & 3) + 4 * half + (r & 3);
+        const uint32_t q = q0 + row;
+        if (q < A.n_queries) out[static_cast<uint64_t>(q) * A.cap + step_index] = v[e];
     }
 }
 
@@ -451,13 +493,23 @@ template <int DT, int CH, int RB, bool kBlockMax>
 int Launch(const ScoreArgs& A, hipStream_t stream, uint32_t grid_x) {
     const size_t lds = static_cast<size_t>(RB) * CH * 64 * sizeof(u32x4) + 2 * 32 * RB * sizeof(float) +
                        kWaves * 32 * RB * sizeof(uint32_t) + kWaves * (kQueueKeys + kQueueRuns) * sizeof(uint64_t);
-    static bool attr_set = false;
-    if (!attr_set) {
-        FRK_HIP_CHECK(hipFuncSetAttribute(reinterpret_cast<const void*>(&ScoreKernel<DT, CH, RB, kBlockMax>),
-                                          hipFuncAttributeMaxDynamicSharedMemorySize, static_cast<int>(lds)));
-        attr_set = true;
+    const void* fn = reinterpret_cast<const void*>(&ScoreKernel<DT, CH, RB, kBlockMax>);
+    static int resident_per_cu = 0;  // workgroups of this shape one CU holds at once
+    if (resident_per_cu == 0) {
+        FRK_HIP_CHECK(hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize, static_cast<int>(lds)));
+        int occ = 0;
+        FRK_HIP_CHECK(hipOccupancyMaxActiveBlocksPerMultiprocessor(&occ, fn, kThreads, lds));
+        resident_per_cu = occ > 0 ? occ : 1;
     }
+    int dev = 0, cus = 0;
+    FRK_HIP_CHECK(hipGetDevice(&dev));
+    FRK_HIP_CHECK(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev));
     const uint32_t groups = (A.n_queries + 32 * RB - 1) / (32 * RB);
+    // one wave of resident workgroups over the chip (each walks a contiguous block range, so the
+    // query tile is staged once per workgroup), never more than the caller's work-based cap
+    const uint32_t resident = static_cast<uint32_t>(cus > 0 ? cus : 1) * static_cast<uint32_t>(resident_per_cu);
+    const uint32_t per_group = resident / groups > 0 ? resident / groups : 1;
+    if (grid_x > per_group) grid_x = per_group;
     hipLaunchKernelGGL((ScoreKernel<DT, CH, RB, kBlockMax>), dim3(grid_x, groups), dim3(kThreads), lds, stream, A);
     FRK_HIP_CHECK(hipGetLastError());
     return FRK_OK;

@@ -50,9 +50,10 @@ def filter_queries_array(descs):
 
 
 def filter_eval(d_queries, n_queries, d_expr, d_table, n_segments, seg_docs, n_docs, d_out, out_stride, d_counts,
-                d_status, stream=None):
+                d_status, max_postings=0, stream=None):
     check(lib().frk_filter_eval(ptr(d_queries), n_queries, ptr(d_expr), ptr(d_table), n_segments, seg_docs, n_docs,
-                                ptr(d_out), out_stride, ptr(d_counts), ptr(d_status), stream_ptr(stream)),
+                                ptr(d_out), out_stride, ptr(d_counts), ptr(d_status), max_postings,
+                                stream_ptr(stream)),
           "frk_filter_eval")
 
 

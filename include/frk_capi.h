@@ -55,6 +55,9 @@ const char* frk_last_error(void);
  * that segment; bits at or past the segment's doc count are 0. counts[q] += set bits (the caller
  * zeroes counts). Bit layout is byte-identical to the reference's per-segment u16 bitsets when
  * seg_docs is a multiple of 64 (required).
+ * max_postings: the largest n_postings of the batch (0 = unknown). It sizes the LDS staging of
+ * operands; a query with more postings is still evaluated exactly, with its extra operands read
+ * straight from HBM.
  * ------------------------------------------------------------------------------------------------ */
 typedef struct {
     uint32_t expr_off;   /* first program word in the batch's expr array */
@@ -72,7 +75,7 @@ typedef struct {
 int frk_filter_eval(const frk_filter_query* d_queries, uint32_t n_queries, const uint32_t* d_expr,
                     const uint64_t* d_table, uint32_t n_segments, uint32_t seg_docs, uint64_t n_docs,
                     uint64_t* d_out, uint64_t out_stride, uint32_t* d_counts, uint32_t* d_status,
-                    frk_stream_t stream);
+                    uint32_t max_postings, frk_stream_t stream);
 
 /* ------------------------------------------------------------------------------------------------
  * Doc store: vectors in the MFMA-native tiled layout. Docs are grouped in blocks of 32; the dims in

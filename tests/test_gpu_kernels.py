@@ -34,7 +34,7 @@ def dev(a, dtype=None):
     return t.cuda()
 
 
-def _filter_case(frk, seed, n_docs, seg_docs, n_tags, n_queries, depth, all_layouts=True):
+def _filter_case(frk, seed, n_docs, seg_docs, n_tags, n_queries, depth, all_layouts=True, max_postings=0):
     print("filter case seed=%d n_docs=%d seg=%d tags=%d q=%d" % (seed, n_docs, seg_docs, n_tags, n_queries),
           flush=True)
     rng = np.random.default_rng(seed)
@@ -77,7 +77,8 @@ def _filter_case(frk, seed, n_docs, seg_docs, n_tags, n_queries, depth, all_layo
     d_out = torch.full((n_queries, stride), -1, dtype=torch.int64, device="cuda")
     d_cnt = torch.zeros(n_queries, dtype=torch.int32, device="cuda")
     d_status = torch.zeros(1, dtype=torch.int32, device="cuda")
-    frk.filter_eval(d_q, n_queries, d_expr, d_tab, n_seg, seg_docs, n_docs, d_out, stride, d_cnt, d_status)
+    frk.filter_eval(d_q, n_queries, d_expr, d_tab, n_seg, seg_docs, n_docs, d_out, stride, d_cnt, d_status,
+                    max_postings)
     torch.cuda.synchronize()
     assert int(d_status.item()) == 0
     out = d_out.cpu().numpy().view(np.uint64)
@@ -97,9 +98,13 @@ def test_filter_bit_exact_all_layouts(frk):
     _filter_case(frk, seed=1, n_docs=3 * 8192 + 1000, seg_docs=8192, n_tags=40, n_queries=60, depth=4)
 
 
-def test_filter_bit_exact_full_segments(frk):
-    # reference segment size (131072 docs = 2048 words, 8 chunks per segment), short tail segment
-    _filter_case(frk, seed=2, n_docs=131072 + 19264, seg_docs=131072, n_tags=24, n_queries=20, depth=3)
+@pytest.mark.parametrize("max_postings", [0, 2, 4, 8])
+def test_filter_bit_exact_full_segments(frk, max_postings):
+    # reference segment size (131072 docs = 2048 words), short tail segment; the launch sizing hint
+    # selects one chunk per segment (4 slots), two (8 slots) or four (16 slots), and a hint below the
+    # programs' posting counts sends the extra operands down the direct path
+    _filter_case(frk, seed=2, n_docs=131072 + 19264, seg_docs=131072, n_tags=24, n_queries=20, depth=3,
+                 max_postings=max_postings)
 
 
 def test_filter_single_segment_tiny(frk):
