@@ -7,6 +7,27 @@
 namespace frk {
 namespace {
 
+// The mask word the scorer reads for every (block, query) when no filter masks are given.
+__device__ uint32_t kAllPassMask[1] = {0xFFFFFFFFu};
+
+int AllPassMask(const uint32_t** out) {
+    constexpr int kMaxDevices = 64;
+    static const uint32_t* addr[kMaxDevices] = {};
+    int dev = 0;
+    FRK_HIP_CHECK(hipGetDevice(&dev));
+    if (dev < 0 || dev >= kMaxDevices) {
+        SetError("device %d out of range", dev);
+        return FRK_ERR_UNSUPPORTED;
+    }
+    if (!addr[dev]) {
+        void* p = nullptr;
+        FRK_HIP_CHECK(hipGetSymbolAddress(&p, HIP_SYMBOL(kAllPassMask)));
+        addr[dev] = static_cast<const uint32_t*>(p);
+    }
+    *out = addr[dev];
+    return FRK_OK;
+}
+
 int Prepare(const frk_doc_store* store, const float* d_queries, uint32_t n_queries, const uint32_t* d_bmask,
             uint32_t bmask_stride, const float* d_floor, uint64_t block_begin, uint64_t block_step,
             uint64_t block_count, uint64_t* d_out, uint32_t cap, uint32_t* d_counts, frk_stream_t stream,
@@ -48,8 +69,14 @@ int Prepare(const frk_doc_store* store, const float* d_queries, uint32_t n_queri
     A.n_queries = n_queries;
     A.dim = store->dim;
     A.q_dim = store->dim;
-    A.bmask = d_bmask;
-    A.bmask_stride = bmask_stride;
+    if (d_bmask) {
+        A.bmask = d_bmask;
+        A.bmask_stride = bmask_stride;
+    } else {
+        int rc = AllPassMask(&A.bmask);
+        if (rc != FRK_OK) return rc;
+        A.bmask_stride = 0;
+    }
     A.floor = d_floor;
     A.block_begin = block_begin;
     A.block_step = block_step;

@@ -92,7 +92,7 @@ struct ScoreArgs {
     uint32_t dim;  // padded dim of the store
     uint32_t q_dim;  // row length of the query matrix (== dim)
     const uint32_t* bmask;    // block-major filter masks: bmask[block * bmask_stride + q], bit j = doc 32 block + j
-    uint32_t bmask_stride;    // >= n_queries; NULL bmask = every doc passes
+    uint32_t bmask_stride;    // >= n_queries; 0 = no filter (bmask then points at one all-pass word)
     const float* floor;
     uint64_t block_begin, block_step, block_count;
     uint64_t* cand;
@@ -146,16 +146,29 @@ __device__ __forceinline__ void StageQueries(const ScoreArgs& A, u32x4* a_lds, f
     }
 }
 
-// Chunks per pipeline group and groups in flight: a wave streams its doc blocks as a flat sequence of
-// (block, group) steps through a ring of kRing register buffers, so kRing - 1 groups of loads are in
-// flight while one group feeds the MFMAs.
-constexpr int kRing = 3;
+// Doc streaming: a wave walks its doc blocks as a flat sequence of (block, group) steps, a group being
+// G = 4 chunks (4 KiB) of one block. A ring of kRing = 4 register buffers keeps the loads of the next
+// three steps in flight while one step feeds the MFMAs. The loop body is unrolled over
+// U = lcm(kRing, groups per block) steps and always starts at a block boundary, so the ring slot and
+// the group index of every step are compile-time constants: the ring stays in registers and the block
+// epilogue appears once per body position that ends a block. Loads are unconditional (a step past
+// the end reloads the last step's addresses), so the code between a load and its use is straight-line
+// and each wait counts the loads still allowed in flight exactly.
+constexpr int kRing = 4;
+
+// f(integral_constant<int, I>) for each I of the sequence, in order.
+template <typename F, int... I>
+__device__ __forceinline__ void StaticFor(F&& f, std::integer_sequence<int, I...>) {
+    (f(std::integral_constant<int, I>{}), ...);
+}
 
 template <int CH>
 struct GroupShape {
-    static constexpr int G = CH < 8 ? CH : 8;  // chunks per group
-    static constexpr int kGroups = CH / G;     // groups per doc block
+    static constexpr int G = CH < 4 ? CH : 4;                          // chunks per group
+    static constexpr int kGroups = CH / G;                             // groups per doc block
+    static constexpr int kUnroll = kGroups >= kRing ? kGroups : kRing;  // steps per loop body
     static_assert(CH % G == 0, "chunks per doc must be a multiple of the group size");
+    static_assert(kUnroll % kGroups == 0 && kUnroll % kRing == 0, "loop body must hold whole blocks and rings");
 };
 
 template <int DT, int CH, int RB>
@@ -186,8 +199,11 @@ __device__ __forceinline__ void ComputeGroup(const u32x4* a_lds, int g, const u3
 // wave-private LDS queue, in runs of one (row, lane half) = one query each; the queue is flushed in bulk
 // (one counter atomic per run, all issued before any result is awaited, then the stores), so no global
 // memory access or atomic sits inside the MFMA loop.
-constexpr int kQueueKeys = 512;  // keys per wave
-constexpr int kQueueRuns = 128;  // runs per wave
+#ifndef FRK_SCORE_QUEUE_KEYS
+#define FRK_SCORE_QUEUE_KEYS 256  // 20 KiB of survivor queue per workgroup would leave room for only one per CU
+#endif
+constexpr int kQueueKeys = FRK_SCORE_QUEUE_KEYS;  // keys per wave
+constexpr int kQueueRuns = kQueueKeys / 4;        // runs per wave
 
 struct WaveQueue {
     uint64_t* keys;  // [kQueueKeys]
@@ -225,18 +241,30 @@ __device__ __forceinline__ void FlushQueue(const ScoreArgs& A, WaveQueue& Qu) {
     Qu.nr = 0;
 }
 
-// The block's filter masks for this wave's row blocks, staged through LDS so each lane can read the
-// 16 rows it holds: lane l fetches query row (l % 32) of each row block (one coalesced 128-byte load).
+// The block's filter masks for this wave's row blocks: lane l loads query row (l % 32) of each row block
+// (one coalesced 128-byte load per row block) a few steps before the block's epilogue; the epilogue
+// stages them through LDS so each lane can read the 16 rows it holds.
 template <int RB>
-__device__ __forceinline__ void StageMasks(const ScoreArgs& A, uint32_t q0, uint64_t blk, uint32_t* mask_lds) {
+__device__ __forceinline__ void LoadMasks(const ScoreArgs& A, uint32_t q0, uint64_t blk, uint32_t (&m)[RB]) {
     const int lane = threadIdx.x & 63;
+    // bmask_stride 0: no filter, bmask is one all-pass word (no branch, so the loads stay in
+    // straight-line code)
+    const uint32_t* base = A.bmask + blk * A.bmask_stride;
 #pragma unroll
     for (int rb = 0; rb < RB; rb++) {
         const uint32_t q = q0 + 32 * rb + (lane & 31);
-        uint32_t m = 0;
-        if (q < A.n_queries) m = A.bmask ? A.bmask[blk * A.bmask_stride + q] : 0xFFFFFFFFu;
-        if (lane < 32) mask_lds[32 * rb + lane] = m;
+        const uint32_t off = A.bmask_stride ? (q < A.n_queries ? q : A.n_queries - 1) : 0u;
+        const uint32_t v = base[off];
+        m[rb] = q < A.n_queries ? v : 0u;
     }
+}
+
+template <int RB>
+__device__ __forceinline__ void StageMasks(const uint32_t (&m)[RB], uint32_t* mask_lds) {
+    const int lane = threadIdx.x & 63;
+#pragma unroll
+    for (int rb = 0; rb < RB; rb++)
+        if (lane < 32) mask_lds[32 * rb + lane] = m[rb];
     WaveSync();
 }
 
@@ -431,40 +459,49 @@ __global__ __launch_bounds__(kThreads, FRK_SCORE_MIN_BLOCKS) void ScoreKernel(Sc
     if (i_lo + wave >= i_hi) return;
     const uint64_t n_mine = (i_hi - i_lo - wave + kWaves - 1) / kWaves;  // blocks of this wave
     const uint64_t total = n_mine * NG;                                   // (block, group) steps
+    constexpr int U = GroupShape<CH>::kUnroll;
 
-    auto src = [&](uint64_t step) -> const u32x4* {
-        const uint64_t jj = step / NG;
-        const uint64_t blk = A.block_begin + (i_lo + wave + jj * kWaves) * A.block_step;
-        return A.tiles + (blk * CH + (step % NG) * G) * 64 + lane;
-    };
+    auto block_index = [&](uint64_t step) -> uint64_t { return i_lo + wave + (step / NG) * kWaves; };
     u32x4 ring[kRing][G];
+    uint32_t mring[kRing][RB];  // mask words of the block whose last group sits in the same slot
+    // load step `step` (group index GL, a compile-time constant) into ring slot SL
+    auto load = [&](auto slot_c, auto group_c, uint64_t step) __attribute__((always_inline)) {
+        constexpr int SL = decltype(slot_c)::value;
+        constexpr int GL = decltype(group_c)::value;
+        const uint64_t st = step < total ? step : total - 1;
+        const uint64_t blk = A.block_begin + block_index(st) * A.block_step;
+        const u32x4* p = A.tiles + (blk * CH + static_cast<uint64_t>(GL) * G) * 64 + lane;
 #pragma unroll
-    for (int s = 0; s < kRing - 1; s++)
-        if (static_cast<uint64_t>(s) < total) {
-            const u32x4* p = src(s);
-#pragma unroll
-            for (int c = 0; c < G; c++) ring[s][c] = p[c * 64];
-        }
+        for (int c = 0; c < G; c++) ring[SL][c] = p[c * 64];
+        if constexpr (GL == NG - 1) LoadMasks<RB>(A, q0, blk, mring[SL]);
+    };
+    // the first kRing - 1 steps, in step order (each wait below counts on that order)
+    load(std::integral_constant<int, 0>{}, std::integral_constant<int, 0 % NG>{}, 0);
+    __builtin_amdgcn_sched_barrier(0);
+    load(std::integral_constant<int, 1>{}, std::integral_constant<int, 1 % NG>{}, 1);
+    __builtin_amdgcn_sched_barrier(0);
+    load(std::integral_constant<int, 2>{}, std::integral_constant<int, 2 % NG>{}, 2);
+    __builtin_amdgcn_sched_barrier(0);
     floatx16 acc[RB];
 #pragma unroll
     for (int rb = 0; rb < RB; rb++)
         for (int r = 0; r < 16; r++) acc[rb][r] = 0.0f;
-    // one pipeline step per ring slot; the slot index is a compile-time constant in each call, so the
-    // ring stays in registers without relying on loop unrolling
-    auto step_fn = [&](auto slot_c, uint64_t step) __attribute__((always_inline)) {
-        constexpr int S = decltype(slot_c)::value;
-        const uint64_t ahead = step + kRing - 1;
-        if (ahead < total) {
-            const u32x4* p = src(ahead);
-#pragma unroll
-            for (int c = 0; c < G; c++) ring[(S + kRing - 1) % kRing][c] = p[c * 64];
-        }
-        const int g = static_cast<int>(step % NG);
+    // step s of the loop body (body starts at a block boundary `base`)
+    auto step_fn = [&](auto s_c, uint64_t base) __attribute__((always_inline)) {
+        constexpr int s = decltype(s_c)::value;
+        constexpr int S = s % kRing;
+        constexpr int g = s % NG;
+        const uint64_t step = base + s;
+        load(std::integral_constant<int, (S + kRing - 1) % kRing>{}, std::integral_constant<int, (s + kRing - 1) % NG>{},
+             step + kRing - 1);
+        // keep the prefetch at the head of the step: the scheduler would otherwise sink the loads
+        // towards their use and shrink the distance the ring is there to provide
+        __builtin_amdgcn_sched_barrier(0);
         ComputeGroup<DT, CH, RB>(a_lds, g, ring[S], acc);
-        if (g == NG - 1) {
-            const uint64_t i = i_lo + wave + (step / NG) * kWaves;
+        if constexpr (g == NG - 1) {
+            const uint64_t i = block_index(step);
             const uint64_t blk = A.block_begin + i * A.block_step;
-            StageMasks<RB>(A, q0, blk, mask_lds);
+            StageMasks<RB>(mring[S], mask_lds);
             if constexpr (kBlockMax)
                 EpilogueBlockMax<DT, RB>(A, q_scale_lds, mask_lds, q0, blk, i, acc);
             else
@@ -475,15 +512,16 @@ __global__ __launch_bounds__(kThreads, FRK_SCORE_MIN_BLOCKS) void ScoreKernel(Sc
                 for (int r = 0; r < 16; r++) acc[rb][r] = 0.0f;
         }
     };
-    static_assert(kRing == 3, "the step sequence below is written out for a ring of 3");
-    uint64_t step = 0;
-    for (; step + 3 <= total; step += 3) {
-        step_fn(std::integral_constant<int, 0>{}, step);
-        step_fn(std::integral_constant<int, 1>{}, step + 1);
-        step_fn(std::integral_constant<int, 2>{}, step + 2);
+    uint64_t base = 0;
+    for (; base + U <= total; base += U)
+        StaticFor([&](auto s_c) __attribute__((always_inline)) { step_fn(s_c, base); },
+                  std::make_integer_sequence<int, U>{});
+    if constexpr (U > NG) {
+        // the last blocks when a body holds several (small blocks): whole blocks, one at a time
+        StaticFor([&](auto s_c) __attribute__((always_inline)) {
+            if (base + decltype(s_c)::value < total) step_fn(s_c, base);
+        }, std::make_integer_sequence<int, U - NG>{});
     }
-    if (step < total) step_fn(std::integral_constant<int, 0>{}, step);
-    if (step + 1 < total) step_fn(std::integral_constant<int, 1>{}, step + 1);
     if constexpr (!kBlockMax) FlushQueue(A, Qu);
 }
 
