@@ -76,41 +76,73 @@ def recall(chosen, plain):
     return hits / max(tot, 1)
 
 
-def cpu_baseline(docs, dens, off, ids, queries, filters, ks, n_docs_full, threads):
+def cpu_baseline(docs, dens, off, ids, queries, filters, ks, n_docs_full, threads, batch):
     """The reference's CPU brute force (CpuTopKBatch, 10.13 recall_ref.cpp:90-140, compiled from the
-    reference sources into oracle/_ref) over a doc sample, threads over disjoint doc ranges; else
-    the oracle's port of it. Scaled to queries/s over the full corpus."""
+    reference sources into oracle/_ref) over the same corpus and query stream, one batch of queries per
+    pass over the docs, threads over disjoint doc ranges; else the oracle's port of it. When `docs`
+    holds only the first rows of the corpus the rate is scaled by docs."""
     import oracle
     from hybridsearchsystem_amd import synth
     sample_docs = docs.shape[0]
     words = synth.doc_tag_words(off, ids, n_docs_full, len(dens), 0, sample_docs)
-    asts = [oracle.parse_filter(f) for f in filters]
     use_ref = oracle.ref() is not None
     per = (sample_docs + threads - 1) // threads
-    out = [None] * threads
 
-    def work(t):
-        lo, hi = t * per, min(sample_docs, (t + 1) * per)
-        if lo >= hi:
-            return
-        if use_ref:
-            out[t] = oracle.ref_cpu_topk_batch(docs, words, queries, asts, ks, fp32=True, doc_offset=lo,
-                                               n_scan=hi - lo)
-        else:
-            masks = np.stack([oracle.filter_bitmap(a, words[lo:hi]) for a in asts])
-            out[t] = oracle.topk(docs[lo:hi], queries, ks, masks, mode=0, threads=1)
+    def one_batch(qb, asts, kb):
+        def work(t):
+            lo, hi = t * per, min(sample_docs, (t + 1) * per)
+            if lo >= hi:
+                return
+            if use_ref:
+                oracle.ref_cpu_topk_batch(docs, words, qb, asts, kb, fp32=True, doc_offset=lo, n_scan=hi - lo)
+            else:
+                masks = np.stack([oracle.filter_bitmap(a, words[lo:hi]) for a in asts])
+                oracle.topk(docs[lo:hi], qb, kb, masks, mode=0, threads=1)
 
+        th = [threading.Thread(target=work, args=(t,)) for t in range(threads)]
+        for x in th:
+            x.start()
+        for x in th:
+            x.join()
+
+    batches = []
+    for lo in range(0, queries.shape[0], batch):
+        sl = slice(lo, lo + batch)
+        batches.append((queries[sl], [oracle.parse_filter(f) for f in filters[sl]], ks[sl]))
     t0 = time.perf_counter()
-    th = [threading.Thread(target=work, args=(t,)) for t in range(threads)]
-    for x in th:
-        x.start()
-    for x in th:
-        x.join()
+    for qb, asts, kb in batches:
+        one_batch(qb, asts, kb)
     secs = time.perf_counter() - t0
     qps = queries.shape[0] / (secs * n_docs_full / sample_docs)
+    scope = "all %d docs" % n_docs_full if sample_docs == n_docs_full else \
+        "first %d of %d docs, scaled by docs" % (sample_docs, n_docs_full)
     return dict(value=round(qps, 3), unit="queries/s", cores=threads, kind="reference" if use_ref else "port",
-                sample="%d queries x first %d of %d docs (%.1fs), scaled by docs" % (queries.shape[0], sample_docs,
-                                                                                      n_docs_full, secs))
+                sample="%d queries in batches of %d x %s, %.1f s" % (queries.shape[0], batch, scope, secs))
+
+
+def traffic_from_profile(path):
+    """HBM bytes per launch of the main score kernel from a committed rocprofv3 counter summary (the
+    line `traffic_bytes_per_launch: N`), or None when there is none."""
+    try:
+        with open(path) as f:
+            for line in f:
+                if line.startswith("traffic_bytes_per_launch:"):
+                    return int(float(line.split(":", 1)[1]))
+    except OSError:
+        pass
+    return None
+
+
+def calibrate_copy(n_bytes=1 << 30):
+    """One device-to-device copy of n_bytes (torch's copy kernel): the reference dispatch for scaling
+    the FETCH_SIZE / WRITE_SIZE counters of a profiled run to bytes."""
+    import torch
+    src = torch.ones(n_bytes // 4, dtype=torch.float32, device="cuda")
+    dst = torch.empty_like(src)
+    torch.cuda.synchronize()
+    dst.copy_(src)
+    torch.cuda.synchronize()
+    print(json.dumps({"calibration": "copy", "bytes": n_bytes}), flush=True)
 
 
 def main():
@@ -123,10 +155,17 @@ def main():
     ap.add_argument("--batch", type=int, default=128)
     ap.add_argument("--k", type=int, default=100)
     ap.add_argument("--queries", type=int, default=1024)
-    ap.add_argument("--cpu-sample", type=int, default=4_000_000)
+    ap.add_argument("--cpu-docs", type=int, default=0, help="docs the CPU baseline scans (0 = all)")
+    ap.add_argument("--cpu-queries", type=int, default=1024, help="queries the CPU baseline answers")
     ap.add_argument("--cpu-threads", type=int, default=16)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--calibrate-copy", action="store_true", help="run one 1 GiB copy and exit")
+    ap.add_argument("--traffic-profile", default=os.path.join(ROOT, "profiles", "score_bf16_10Mx256_q128_traffic.md"),
+                    help="rocprofv3 counter summary holding the main score kernel's HBM bytes per launch")
     args = ap.parse_args()
+    if args.calibrate_copy:
+        calibrate_copy()
+        return
     if args.gpus != 1:
         print("bench.py runs on one GPU: this build has no multi-GPU path (--gpus %d)" % args.gpus, file=sys.stderr)
         sys.exit(2)
@@ -135,8 +174,9 @@ def main():
     from hybridsearchsystem_amd.search import FRK_DTYPE_BF16, FRK_DTYPE_F32
 
     n_tags, centroids, seed = 1024, 1024, synth.SEEDS["corpus"]
+    cpu_docs = min(args.cpu_docs or args.docs, args.docs)
     table, dens, off, ids, head = build_table(args.docs, args.dim, n_tags, centroids, (FRK_DTYPE_BF16,), seed,
-                                              0 if args.no_cpu else min(args.cpu_sample, args.docs))
+                                              0 if args.no_cpu else cpu_docs)
     queries = synth.mixture_queries(synth.SEEDS["queries"], args.queries, args.dim, centroids, corpus_seed=seed)
     filters, sels = synth.random_filters(synth.SEEDS["filters"], dens, args.queries, depth=3, sel=(0.0001, 0.6))
     ks = np.full(args.queries, args.k, np.uint32)
@@ -163,17 +203,24 @@ def main():
     lat = lat_c if chosen_ok else lat_p
     st = st_c if chosen_ok else st_p
 
-    # roofline of the dominant kernel: the main fused score pass (HIP events around its launch)
+    # roofline of the dominant kernel: the main fused score pass, timed by HIP events recorded on the
+    # searcher's stream around its launch. Algorithmic bytes per launch: the doc stream once, the
+    # block-major filter masks (4 bytes per (32-doc block, query row of the padded batch)), and the
+    # 8-byte candidate keys written.
     n, d, q = args.docs, args.dim, args.batch
-    doc_bytes = ((n + 31) // 32) * 32 * d * (2 if chosen_ok else 4)
-    mask_bytes = q * ((n + 63) // 64) * 8
-    alg_bytes = doc_bytes + mask_bytes
+    n_blocks = (n + 31) // 32
+    doc_bytes = n_blocks * 32 * d * (2 if chosen_ok else 4)
+    mask_bytes = n_blocks * ((q + 31) // 32 * 32) * 4
+    cand_bytes = float(np.mean([s["candidates"] for s in st])) * 8
+    alg_bytes = doc_bytes + mask_bytes + cand_bytes
     ms_score = float(np.mean([s["ms_score"] for s in st]))
     achieved = alg_bytes / (ms_score * 1e-3) / 1e9
     roof = dict(bound="hbm", achieved=round(achieved, 1), peak=HBM_PEAK_GBS, unit="GB/s",
-                frac=round(achieved / HBM_PEAK_GBS, 4), traffic=None)
+                frac=round(achieved / HBM_PEAK_GBS, 4), traffic=traffic_from_profile(args.traffic_profile),
+                alg_bytes=int(alg_bytes), kernel_ms=round(ms_score, 4))
     ms_score_p = float(np.mean([s["ms_score"] for s in st_p]))
     flops_p = 2.0 * q * n * d
+    tf_p = flops_p / (ms_score_p * 1e-3) / 1e12
     out = {
         "metric": METRIC, "value": round(value, 2), "unit": "queries/s", "n_gpus": 1, "steps": args.steps,
         "warmup": args.warmup, "ms_per_step": round(wall_c / args.steps * 1e3 if chosen_ok else wall_p / args.steps * 1e3, 3),
@@ -185,7 +232,8 @@ def main():
         "recall_at_k_vs_plain": round(rec, 5),
         "plain_fp32": {"qps": round(qps_p, 2), "p50_ms": round(float(np.percentile(lat_p, 50)), 3),
                        "p99_ms": round(float(np.percentile(lat_p, 99)), 3), "score_kernel_ms": round(ms_score_p, 3),
-                       "score_tflops": round(flops_p / (ms_score_p * 1e-3) / 1e12, 1)},
+                       "roofline": {"bound": "mfma", "achieved": round(tf_p, 1), "peak": MFMA_PEAK_TFLOPS["f32"],
+                                    "unit": "TFLOP/s", "frac": round(tf_p / MFMA_PEAK_TFLOPS["f32"], 4)}},
         "chosen": {"qps": round(qps_c, 2), "pipeline": "bf16 + probe floor + fp32 rescore of top k*4",
                    "speedup_vs_plain": round(qps_c / qps_p, 3)},
         "stages_ms": {k: round(float(np.mean([s[k] for s in st])), 3) for k in
@@ -196,8 +244,10 @@ def main():
         "roofline": roof,
     }
     if not args.no_cpu:
-        out["cpu_baseline"] = cpu_baseline(head, dens, off, ids, queries[:args.batch], filters[:args.batch],
-                                           ks[:args.batch], n, args.cpu_threads)
+        nq = min(args.cpu_queries, args.queries)
+        log("cpu baseline: %d queries over %d docs, %d threads" % (nq, head.shape[0], args.cpu_threads))
+        out["cpu_baseline"] = cpu_baseline(head, dens, off, ids, queries[:nq], filters[:nq], ks[:nq], n,
+                                           args.cpu_threads, args.batch)
     print(json.dumps(out), flush=True)
 
 

@@ -3,15 +3,15 @@
  *
  * Two layers, both plain C (no C++ or torch types):
  *
- *  1. Kernel entry points (frk_filter_eval, frk_score_select, frk_select_topk, frk_rescore,
- *     frk_mlp_*). They replace the reference's five AscendC launch stubs, which the host stages call
- *     through ACLRT_LAUNCH_KERNEL (each entry cites the kernel it replaces). Stream-ordered, no
- *     implicit synchronisation, no allocation; the caller owns every device buffer. Reentrant
- *     across streams.
+ *  1. Kernel entry points (frk_filter_eval, frk_masks_to_blocks, frk_score_select,
+ *     frk_score_blockmax, frk_kth_largest, frk_select_topk, frk_rescore). They replace the
+ *     reference's five AscendC launch stubs, which the host stages call through
+ *     ACLRT_LAUNCH_KERNEL (each entry cites the kernel it replaces). Stream-ordered, no implicit
+ *     synchronisation, no allocation; the caller owns every device buffer. Reentrant across streams.
  *
- *  2. Index and searcher handles (frk_table_*, frk_searcher_*), a C face of the C++
- *     NpuRetrieval::DataTable / FullRecallSearcher classes (hybridsearchsystem_amd/csrc/engine/),
- *     for callers that cannot link C++ (ctypes, cgo, JNI).
+ *  2. Index and search handles (frk_table_*, frk_builder_*, frk_search, filter codecs), a C face of
+ *     the C++ NpuRetrieval::DataTable / FullRecallSearcher classes (hybridsearchsystem_amd/csrc/
+ *     engine/), for callers that cannot link C++ (ctypes, cgo, JNI).
  *
  * Every function returns 0 (FRK_OK) or a negative FRK_ERR_* code; frk_last_error() gives the text
  * of the calling thread's last failure.
