@@ -4,7 +4,7 @@
  * Two layers, both plain C (no C++ or torch types):
  *
  *  1. Kernel entry points (frk_filter_eval, frk_masks_to_blocks, frk_score_select,
- *     frk_score_blockmax, frk_kth_largest, frk_select_topk, frk_rescore). They replace the
+ *     frk_score_blockmax, frk_kth_largest, frk_select_topk, frk_rescore, frk_mlp_*). They replace the
  *     reference's five AscendC launch stubs, which the host stages call through
  *     ACLRT_LAUNCH_KERNEL (each entry cites the kernel it replaces). Stream-ordered, no implicit
  *     synchronisation, no allocation; the caller owns every device buffer. Reentrant across streams.
@@ -171,6 +171,53 @@ int frk_select_topk(const uint64_t* d_cand, uint32_t cand_cap, const uint32_t* d
  * ------------------------------------------------------------------------------------------------ */
 int frk_rescore(const frk_doc_store* f32_store, const float* d_queries, uint32_t n_queries, const uint64_t* d_keys_in,
                 uint32_t stride, const uint32_t* d_counts, uint64_t* d_keys_out, frk_stream_t stream);
+
+/* ------------------------------------------------------------------------------------------------
+ * Learned score floor: BF16 training and inference kernels of a small MLP
+ * (in_dim -> hidden -> hidden -> 1, ReLU) that predicts a lower bound on a query's k-th best filtered
+ * score from the query vector and filter features; the searcher uses it as a per-query floor
+ * (frk_search_options.floors) and reruns unpruned any query it would cut below k survivors. No
+ * reference counterpart (the reference has no learned component). Every GEMM rounds its operands to
+ * BF16 (nearest even) and accumulates in FP32 on MFMA; tensors are FP32 row-major in device memory.
+ * ------------------------------------------------------------------------------------------------ */
+#define FRK_MLP_EPI_NONE 0      /* C = op(A) op(B) */
+#define FRK_MLP_EPI_BIAS_RELU 1 /* C = op(A) op(B) + bias (pre-activation), H = max(C, 0) */
+#define FRK_MLP_EPI_RELU_GRAD 2 /* C = Z > 0 ? op(A) op(B) : 0 */
+
+/* C[M][N] = op(A)[M][K] op(B)[K][N]; op(A)(m,k) = trans_a ? A[k*lda + m] : A[m*lda + k],
+ * op(B)(k,n) = trans_b ? B[n*ldb + k] : B[k*ldb + n]. */
+int frk_mlp_gemm(int32_t M, int32_t N, int32_t K, const float* d_a, int32_t lda, int32_t trans_a, const float* d_b,
+                 int32_t ldb, int32_t trans_b, float* d_c, int32_t ldc, int32_t epilogue, const float* d_bias,
+                 const float* d_z, int32_t ldz, float* d_h, int32_t ldh, frk_stream_t stream);
+/* y[r] = sum_j bf16(H[r][j]) bf16(w[j]) + bias[0] (bias may be NULL). */
+int frk_mlp_head_forward(int32_t rows, int32_t dim, const float* d_h, int32_t ldh, const float* d_w,
+                         const float* d_bias, float* d_y, frk_stream_t stream);
+/* dZ[r][j] = Z[r][j] > 0 ? dy[r] bf16(w[j]) : 0. */
+int frk_mlp_head_backward(int32_t rows, int32_t dim, const float* d_dy, const float* d_w, const float* d_z,
+                          int32_t ldz, float* d_dz, int32_t lddz, frk_stream_t stream);
+/* out[j] = sum_r (weights ? weights[r] : 1) x[r][j], x rounded to BF16 when round_bf16. */
+int frk_mlp_column_sum(int32_t rows, int32_t cols, const float* d_x, int32_t ldx, const float* d_weights,
+                       int32_t round_bf16, float* d_out, frk_stream_t stream);
+/* Pinball loss at level tau of r = label - y, mean over rows: loss[r] (optional) and dy[r]. */
+int frk_mlp_pinball(int32_t rows, const float* d_y, const float* d_label, float tau, float* d_dy, float* d_loss,
+                    frk_stream_t stream);
+/* One bias-corrected Adam step (step >= 1) over n FP32 parameters. */
+int frk_mlp_adam(int64_t n, float* d_param, const float* d_grad, float* d_m, float* d_v, float lr, float beta1,
+                 float beta2, float eps, int32_t step, frk_stream_t stream);
+
+typedef struct {
+    const float* w1; /* [hidden][in_dim] */
+    const float* b1; /* [hidden] */
+    const float* w2; /* [hidden][hidden] */
+    const float* b2; /* [hidden] */
+    const float* w3; /* [hidden] */
+    const float* b3; /* [1] */
+    int32_t in_dim, hidden;
+} frk_mlp_params;
+
+/* Inference: y[r] for the rows of x [rows][ldx]; d_work holds 4 * rows * hidden floats. */
+int frk_mlp_forward(const frk_mlp_params* p, int32_t rows, const float* d_x, int32_t ldx, float* d_work, float* d_y,
+                    frk_stream_t stream);
 
 /* ================================================================================================
  * Layer 2: index and searcher handles (C face of NpuRetrieval::DataTable / FullRecallSearcher).
