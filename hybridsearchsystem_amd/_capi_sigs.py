@@ -7,7 +7,7 @@ cp = ctypes.c_char_p
 
 class SearchOptions(ctypes.Structure):
     _fields_ = [("dtype", u32), ("rescore_factor", u32), ("probe", ctypes.c_int32), ("probe_step", u32),
-                ("floors", vp), ("cand_cap", u32)]
+                ("floors", vp), ("cand_cap", u32), ("floor_model", vp)]
 
 
 class SearchStats(ctypes.Structure):
@@ -62,6 +62,8 @@ def bind(L):
     L.frk_compile_filter.restype = i32
     L.frk_compile_filter.argtypes = [vp, cp, cp, vp, ctypes.c_int32, ctypes.POINTER(u32), ctypes.POINTER(u32),
                                      ctypes.POINTER(ctypes.c_int32)]
+    L.frk_floor_features.restype = i32
+    L.frk_floor_features.argtypes = [vp, cp, u32, vp, vp, vp, u32, vp]
     L.frk_search.restype = i32
     L.frk_search.argtypes = [vp, cp, cp, u32, vp, u32, vp, vp, ctypes.POINTER(SearchOptions), u32, vp, vp, vp, vp,
                              ctypes.POINTER(SearchStats)]

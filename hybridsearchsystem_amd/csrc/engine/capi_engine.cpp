@@ -176,6 +176,29 @@ int frk_compile_filter(const frk_table* t, const char* field, const char* expr, 
     return static_cast<int>(fe.postExpr.size());
 }
 
+int frk_floor_features(const frk_table* t, const char* posting_field, uint32_t n, const char* const* filters,
+                       const uint32_t* ks, const float* queries, uint32_t dim, float* out) {
+    if (!t || !posting_field || !ks || !out || (n && !queries))
+        return Fail(FRK_ERR_INVALID, "frk_floor_features: null argument");
+    for (uint32_t i = 0; i < n; i++) {
+        bool ok = true;
+        std::unique_ptr<FilterAst> ast;
+        if (filters && filters[i]) {
+            ast = ParseFilter(filters[i], ok);
+            if (!ok) {
+                frk::SetError("filter %u does not parse: %s", i, filters[i]);
+                return FRK_ERR_INVALID;
+            }
+        }
+        auto tree = BuildQueryNode(ast.get(), posting_field);
+        FilterExpr fe;  // adjusts the tree exactly as a search would
+        if (!PrepareFilterExpr(*t->table, tree.get(), fe)) return FRK_ERR_INVALID;
+        FloorFeatures(*t->table, fe.emptyTree ? nullptr : tree.get(), ks[i], queries + static_cast<size_t>(i) * dim,
+                      dim, out + static_cast<size_t>(i) * kFloorFeatures);
+    }
+    return FRK_OK;
+}
+
 int frk_search(const frk_table* t, const char* vector_field, const char* posting_field, uint32_t n_queries,
                const float* queries, uint32_t dim, const char* const* filters, const uint32_t* ks,
                const frk_search_options* opts, uint32_t k_stride, int64_t* out_ids, float* out_scores,
@@ -206,6 +229,7 @@ int frk_search(const frk_table* t, const char* vector_field, const char* posting
         o.probeStep = opts->probe_step;
         o.floors = opts->floors;
         o.candCap = opts->cand_cap;
+        o.floorModel = opts->floor_model;
         searcher.SetOptions(o);
     }
     std::vector<AggrDeviceResult> dev;

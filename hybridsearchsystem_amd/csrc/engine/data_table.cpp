@@ -58,7 +58,15 @@ bool PostingFieldData::AddPosting(uint64_t token, uint32_t segment, const uint8_
     auto& types = m_types[token];
     if (types.empty()) types.assign(m_segmentNum, 0xFF);
     types[segment] = static_cast<uint8_t>(bytes[3] >> 4);
+    uint32_t head = 0;
+    std::memcpy(&head, bytes, 4);
+    m_hits[token] += head & 0x0FFFFFFFu;  // [layout:4][hitCount:28]
     return true;
+}
+
+uint64_t PostingFieldData::TokenHits(uint64_t token) const {
+    auto it = m_hits.find(token);
+    return it == m_hits.end() ? 0 : it->second;
 }
 
 bool PostingFieldData::FinishAdd(int32_t deviceId) {

@@ -31,6 +31,8 @@ class PostingFieldData {
     // Host staging: the serialized bytes of one posting (head + payload).
     bool AddPosting(uint64_t token, uint32_t segment, const uint8_t* bytes, size_t size);
     bool FinishAdd(int32_t deviceId);  // one H2D of the arena
+    // Docs that carry the token, summed over segments (the hitCount of each posting head).
+    uint64_t TokenHits(uint64_t token) const;
 
     // Per-segment device addresses of a token's postings (0 = all-zero); the shared all-zero vector
     // for an absent token (tokenExist = false), as posting_field_data.cpp:487-495.
@@ -55,6 +57,7 @@ class PostingFieldData {
     std::vector<Pending> m_pending;
     std::unordered_map<uint64_t, std::vector<uint64_t>> m_addrs;
     std::unordered_map<uint64_t, std::vector<uint8_t>> m_types;
+    std::unordered_map<uint64_t, uint64_t> m_hits;
     std::vector<uint64_t> m_zeroAddrs;
     std::vector<uint8_t> m_zeroTypes;
     uint8_t* m_device = nullptr;

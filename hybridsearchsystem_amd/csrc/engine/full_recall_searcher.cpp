@@ -3,6 +3,8 @@
 
 #include <algorithm>
 #include <cmath>
+#include <limits>
+#include <cmath>
 #include <cstring>
 #include <mutex>
 
@@ -75,7 +77,7 @@ class SearchWorkspace {
 
     hipStream_t stream = nullptr;
     hipEvent_t ev[8] = {};
-    Buf batch, bitmaps, bmasks, counters, cand, probe, lp, rescored, out, hbatch, hout;
+    Buf batch, bitmaps, bmasks, counters, cand, probe, lp, rescored, out, mlp, hbatch, hout;
 
    private:
     static std::mutex& PoolMutex() {
@@ -248,7 +250,19 @@ bool FullRecallSearcher::RunDevice(const VectorFieldData& vf, std::vector<AggrDe
     const size_t oK = oQF32 + (lowp ? Pad<float>(static_cast<size_t>(Q) * dpF32) : 0);
     const size_t oKm = oK + Pad<uint32_t>(Q);
     const size_t oFloor = oKm + Pad<uint32_t>(Q);
-    const size_t batchBytes = oFloor + Pad<float>(Q);
+    // learned floors: model input rows [query | standardized features] and per-query offsets
+    const frk_floor_model* model = m_options.floorModel;
+    const bool learned = model != nullptr && m_options.floors == nullptr;
+    const uint32_t inDim = learned ? static_cast<uint32_t>(model->mlp.in_dim) : 0;
+    if (learned && (inDim != m_dim + kFloorFeatures || model->mlp.hidden <= 0 || !model->feat_mean ||
+                    !model->feat_std)) {
+        FRK_LOG_ERROR("floor model expects " << model->mlp.in_dim << " inputs, queries have " << m_dim << " + "
+                                             << kFloorFeatures << " features");
+        return false;
+    }
+    const size_t oX = oFloor + Pad<float>(Q);
+    const size_t oOff = oX + (learned ? Pad<float>(static_cast<size_t>(Q) * inDim) : 0);
+    const size_t batchBytes = oOff + (learned ? Pad<float>(Q) : 0);
     std::unique_ptr<SearchWorkspace> ws = SearchWorkspace::Acquire(m_deviceId);
     struct Releaser {
         std::unique_ptr<SearchWorkspace>& w;
@@ -276,6 +290,24 @@ bool FullRecallSearcher::RunDevice(const VectorFieldData& vf, std::vector<AggrDe
     }
     const bool callerFloor = m_options.floors != nullptr;
     if (callerFloor) std::memcpy(hb + oFloor, m_options.floors, sizeof(float) * Q);
+    if (learned) {
+        float* hx = reinterpret_cast<float*>(hb + oX);
+        float* hoff = reinterpret_cast<float*>(hb + oOff);
+        float f[kFloorFeatures];
+        for (uint32_t q = 0; q < Q; q++) {
+            const float* qv = &m_queries[static_cast<size_t>(q) * m_dim];
+            QueryNode* tree = (descs[q].flags & FRK_FILTER_MATCH_ALL) ? nullptr : m_trees[q];
+            FloorFeatures(*m_table, tree, m_topKs[q] * m, qv, m_dim, f);
+            float* row = hx + static_cast<size_t>(q) * inDim;
+            std::memcpy(row, qv, sizeof(float) * m_dim);
+            for (uint32_t j = 0; j < kFloorFeatures; j++) row[m_dim + j] = (f[j] - model->feat_mean[j]) / model->feat_std[j];
+            // too few docs expected to pass for a floor to pay off (f[2] = log10 expected passing)
+            const bool allow = std::pow(10.0, static_cast<double>(f[2])) >=
+                               static_cast<double>(model->min_passing_ratio) * m_topKs[q] * m;
+            hoff[q] = allow ? -model->margin : -std::numeric_limits<float>::infinity();
+        }
+    }
+    const bool floorChecked = callerFloor || learned;  // a floor that may cut below k: verify, rerun
 
     // ---- device buffers
     const uint64_t nBlocks = (static_cast<uint64_t>(nDocs) + 31) / 32;
@@ -287,12 +319,12 @@ bool FullRecallSearcher::RunDevice(const VectorFieldData& vf, std::vector<AggrDe
     // a query whose k*m exceeds the sampled blocks gets a -inf floor (no pruning), which stays exact
     uint32_t kSelMin = kSel;
     for (uint32_t k : m_topKs) kSelMin = std::min(kSelMin, k * m);
-    const bool useProbe = m_options.probe && !callerFloor && probeStep > 1 && nBlocks / probeStep >= kSelMin;
+    const bool useProbe = m_options.probe && !floorChecked && probeStep > 1 && nBlocks / probeStep >= kSelMin;
     uint32_t cap = m_options.candCap;
     if (cap == 0) {
         cap = useProbe ? static_cast<uint32_t>(std::max<uint64_t>(65536, 16ull * kSel * probeStep))
                        : static_cast<uint32_t>(std::min<uint64_t>(nDocs, std::max<uint64_t>(65536, 64ull * kSel)));
-        if (callerFloor) cap = static_cast<uint32_t>(std::max<uint64_t>(cap, 65536));
+        if (floorChecked) cap = static_cast<uint32_t>(std::max<uint64_t>(cap, 65536));
         cap = std::min<uint32_t>(cap, std::max<uint32_t>(nDocs, 1));
     }
     const uint64_t probeBlocks = useProbe ? (nBlocks + probeStep - 1) / probeStep : 0;
@@ -307,6 +339,7 @@ bool FullRecallSearcher::RunDevice(const VectorFieldData& vf, std::vector<AggrDe
         !ws->cand.Ensure(sizeof(uint64_t) * Q * cap) || !ws->out.Ensure(sizeof(uint64_t) * Q * kmax) ||
         (useProbe && !ws->probe.Ensure(sizeof(float) * Q * probeBlocks)) ||
         (lowp && (!ws->lp.Ensure(sizeof(uint64_t) * Q * kSel) || !ws->rescored.Ensure(sizeof(uint64_t) * Q * kSel))) ||
+        (learned && !ws->mlp.Ensure(sizeof(float) * (4ull * Q * static_cast<uint32_t>(model->mlp.hidden) + 2ull * Q))) ||
         !ws->hout.Ensure(sizeof(uint64_t) * Q * kmax + sizeof(uint32_t) * nCounters))
         return false;
     uint32_t* dCnt = ws->counters.As<uint32_t>();
@@ -345,6 +378,17 @@ bool FullRecallSearcher::RunDevice(const VectorFieldData& vf, std::vector<AggrDe
                             static_cast<uint32_t>(probeBlocks), Q, dKm, 0, dKth, s) != 0)
             return fail("probe select");
         dFloor = dKth;
+    }
+    if (learned) {
+        // the floor model on this stream: prediction - margin, or -inf where no floor applies
+        float* work = ws->mlp.As<float>();
+        float* y = work + 4ull * Q * static_cast<uint32_t>(model->mlp.hidden);
+        float* fl = y + Q;
+        if (frk_mlp_forward(&model->mlp, static_cast<int32_t>(Q), reinterpret_cast<const float*>(db + oX),
+                            static_cast<int32_t>(inDim), work, y, s) != 0 ||
+            frk_mlp_floors(static_cast<int32_t>(Q), y, reinterpret_cast<const float*>(db + oOff), fl, s) != 0)
+            return fail("floor model");
+        dFloor = fl;
     }
     (void)hipEventRecord(ws->ev[2], s);
     if (frk_score_select(store, dQMain, Q, dBMask, qStride, dFloor, 0, 1, nBlocks, ws->cand.As<uint64_t>(), cap,
@@ -396,7 +440,8 @@ bool FullRecallSearcher::RunDevice(const VectorFieldData& vf, std::vector<AggrDe
     m_stats.ms_total = ms(0, 6);
     m_stats.cap = cap;
 
-    // ---- exactness checks: overflowed candidate lists, and a caller floor that cut below the k-th
+    // ---- exactness checks: overflowed candidate lists, and a caller or learned floor that cut below
+    // the k-th
     std::vector<uint32_t> redo;
     uint32_t needCap = cap;
     for (uint32_t q = 0; q < Q; q++) {
@@ -404,33 +449,57 @@ bool FullRecallSearcher::RunDevice(const VectorFieldData& vf, std::vector<AggrDe
         m_stats.candidates += hCnt[cCand + q];
         const uint32_t want = std::min(m_topKs[q] * m, m_passing[q]);
         const bool overflow = hCnt[cCand + q] > cap;
-        const bool shortFloor = callerFloor && std::min(hCnt[cCand + q], cap) < want;
+        const bool shortFloor = floorChecked && std::min(hCnt[cCand + q], cap) < want;
         if (overflow) m_stats.overflowed++;
         if (overflow || shortFloor) {
             redo.push_back(q);
             needCap = std::max(needCap, std::min(hCnt[cCand + q], m_passing[q]));
         }
     }
-    if (!redo.empty()) {
-        // rerun the batch without the caller floor (probe floor or none) and room for every survivor
-        m_stats.reruns = static_cast<uint32_t>(redo.size());
-        SearchOptions saved = m_options;
-        BatchStats first = m_stats;
-        m_options.floors = nullptr;
-        m_options.candCap = std::max<uint32_t>(needCap, cap);
-        if (callerFloor) m_options.probe = true;
-        const bool ok = RunDevice(vf, out);
-        m_options = saved;
-        first.reruns = static_cast<uint32_t>(redo.size());
-        first.ms_total += m_stats.ms_total;
-        m_stats = first;
-        return ok;
-    }
     for (uint32_t q = 0; q < Q; q++) {
         const uint32_t n = hCnt[cOut + q];
         out[q].keys.assign(hKeys + static_cast<size_t>(q) * kmax, hKeys + static_cast<size_t>(q) * kmax + n);
         out[q].passing = m_passing[q];
     }
+    if (redo.empty()) return true;
+
+    // ---- rerun only the affected queries: no caller or learned floor (the exact probe floor instead)
+    // and room for every survivor; their results replace the first pass's
+    m_stats.reruns = static_cast<uint32_t>(redo.size());
+    const BatchStats first = m_stats;
+    const std::vector<uint32_t> passing = m_passing;
+    std::vector<QueryNode*> trees;
+    std::vector<float> queries;
+    std::vector<uint32_t> ks;
+    for (uint32_t q : redo) {
+        trees.push_back(m_trees[q]);
+        queries.insert(queries.end(), m_queries.begin() + static_cast<size_t>(q) * m_dim,
+                       m_queries.begin() + static_cast<size_t>(q + 1) * m_dim);
+        ks.push_back(m_topKs[q]);
+    }
+    std::swap(trees, m_trees);
+    std::swap(queries, m_queries);
+    std::swap(ks, m_topKs);
+    const SearchOptions saved = m_options;
+    m_options.floors = nullptr;
+    m_options.floorModel = nullptr;
+    m_options.candCap = std::max<uint32_t>(needCap, cap);
+    if (floorChecked) m_options.probe = true;
+    // the batch's workspace goes back to the pool first, so the rerun reuses it
+    SearchWorkspace::Release(std::move(ws));
+    std::vector<AggrDeviceResult> sub;
+    const bool ok = RunDevice(vf, sub);
+    m_options = saved;
+    std::swap(trees, m_trees);
+    std::swap(queries, m_queries);
+    std::swap(ks, m_topKs);
+    if (!ok) return false;
+    for (size_t i = 0; i < redo.size(); i++) out[redo[i]] = std::move(sub[i]);
+    BatchStats merged = first;
+    merged.ms_total += m_stats.ms_total;
+    merged.candidates += m_stats.candidates;
+    m_stats = merged;
+    m_passing = passing;
     return true;
 }
 

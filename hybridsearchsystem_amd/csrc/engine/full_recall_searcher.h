@@ -54,6 +54,8 @@ struct SearchOptions {
     const float* floors = nullptr;   // optional caller floor per query (e.g. a learned bound); the
                                      // batch reruns unpruned for any query left with < k survivors
     uint32_t candCap = 0;            // candidate capacity per query; 0 = automatic
+    const frk_floor_model* floorModel = nullptr;  // learned floors, predicted on the device per batch
+                                                  // (same rerun rule as caller floors)
 };
 
 struct BatchStats {

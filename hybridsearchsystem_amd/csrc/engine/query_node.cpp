@@ -1,6 +1,8 @@
 // query_node.cpp -- query tree, postfix compilation and the filter grammar (see query_node.h).
 #include "query_node.h"
 
+#include <algorithm>
+#include <cmath>
 #include <mutex>
 
 #include "../../../include/frk_capi.h"
@@ -239,6 +241,102 @@ uint32_t PostExprMaxStack(const std::vector<uint32_t>& e) {
         mx = sp > mx ? sp : mx;
     }
     return static_cast<uint32_t>(mx);
+}
+
+// ------------------------------------------------------------------------------ floor features
+namespace {
+double TermDensity(const DataTable& t, const std::string& field, uint64_t token) {
+    PostingFieldData* fd = nullptr;
+    if (!t.GetPostingFieldData(field, fd) || fd == nullptr || t.GetDocNum() == 0) return 0.0;
+    return static_cast<double>(fd->TokenHits(token)) / static_cast<double>(t.GetDocNum());
+}
+}  // namespace
+
+void QueryNode::Estimate(const DataTable& t, FilterStats& s) const {
+    const QueryNodeType ty = GetNodeType();
+    double prod = 1.0;  // AND: product of selectivities; OR: product of complements
+    uint32_t depth = 0;
+    bool first = true;
+    double firstSel = 0.0;
+    for (const auto& c : m_children) {
+        FilterStats cs;
+        c->Estimate(t, cs);
+        if (first) firstSel = cs.selectivity;
+        first = false;
+        prod *= (ty == QueryNodeType::OrNode || ty == QueryNodeType::OrTermsNode) ? 1.0 - cs.selectivity
+                                                                                   : cs.selectivity;
+        depth = std::max(depth, cs.depth);
+        s.nAnd += cs.nAnd;
+        s.nOr += cs.nOr;
+        s.nNot += cs.nNot;
+        s.nTerms += cs.nTerms;
+        s.termDensities.insert(s.termDensities.end(), cs.termDensities.begin(), cs.termDensities.end());
+    }
+    s.depth = depth + 1;
+    if (ty == QueryNodeType::NotNode) {
+        s.selectivity = 1.0 - firstSel;
+        s.nNot++;
+    } else if (ty == QueryNodeType::OrNode || ty == QueryNodeType::OrTermsNode) {
+        s.selectivity = 1.0 - prod;
+        s.nOr++;
+    } else {  // And, Conjunction (an AND of OR groups)
+        s.selectivity = prod;
+        s.nAnd++;
+    }
+}
+
+void TermNode::Estimate(const DataTable& t, FilterStats& s) const {
+    const double d = TermDensity(t, m_fieldName, m_token);
+    s.selectivity = d;
+    s.depth = 1;
+    s.nTerms = 1;
+    s.termDensities.assign(1, d);
+}
+
+void TermsNode::Estimate(const DataTable& t, FilterStats& s) const {
+    double comp = 1.0;
+    s.termDensities.clear();
+    if (m_tokens)
+        for (uint64_t tok : *m_tokens) {
+            const double d = TermDensity(t, m_fieldName, tok);
+            comp *= 1.0 - d;
+            s.termDensities.push_back(d);
+        }
+    s.selectivity = 1.0 - comp;
+    s.depth = 2;
+    s.nOr = 1;
+    s.nTerms = static_cast<uint32_t>(s.termDensities.size());
+}
+
+void FloorFeatures(const DataTable& table, QueryNode* tree, uint32_t k, const float* query, uint32_t dim,
+                   float* out) {
+    std::fill(out, out + kFloorFeatures, 0.0f);
+    FilterStats s;
+    const bool matchAll = tree == nullptr || (!tree->IsLeaf() && tree->GetChildrenNum() == 0);
+    if (matchAll) {
+        out[14] = 1.0f;
+    } else {
+        std::shared_lock<std::shared_mutex> lock(tree->m_adjustNodeMutex);
+        tree->Estimate(table, s);
+    }
+    const double sel = std::min(std::max(s.selectivity, 1e-9), 1.0);
+    const double passing = std::max(sel * static_cast<double>(table.GetDocNum()), 1.0);
+    out[0] = static_cast<float>(std::log10(sel));
+    out[1] = static_cast<float>(std::log10(static_cast<double>(std::max<uint32_t>(k, 1))));
+    out[2] = static_cast<float>(std::log10(passing));
+    out[3] = std::min(std::max(out[1] - out[2], -9.0f), 3.0f);
+    out[4] = static_cast<float>(s.depth);
+    out[5] = static_cast<float>(s.nAnd);
+    out[6] = static_cast<float>(s.nOr);
+    out[7] = static_cast<float>(s.nNot);
+    out[8] = static_cast<float>(s.nTerms);
+    std::vector<double> rare;
+    for (double d : s.termDensities) rare.push_back(std::log10(std::max(d, 1e-9)));
+    std::sort(rare.begin(), rare.end());
+    for (size_t i = 0; i < rare.size() && i < 4; i++) out[9 + i] = static_cast<float>(rare[i]);
+    double norm = 0.0;
+    for (uint32_t d = 0; d < dim && query; d++) norm += static_cast<double>(query[d]) * query[d];
+    out[13] = static_cast<float>(std::sqrt(norm));
 }
 
 bool PrepareFilterExpr(const DataTable& table, QueryNode* tree, FilterExpr& out) {

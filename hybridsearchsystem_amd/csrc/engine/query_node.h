@@ -25,6 +25,14 @@ enum FilterOpType : uint32_t { CONJUNCTION = 0, AND = 1, OR = 2, NOT = 3 };
 
 using PostingOperands = std::vector<const std::vector<uint64_t>*>;
 
+// What the learned score floor knows of a filter: its selectivity estimated under tag independence
+// from the table's per-token doc counts, and its shape.
+struct FilterStats {
+    double selectivity = 1.0;
+    uint32_t depth = 0, nAnd = 0, nOr = 0, nNot = 0, nTerms = 0;
+    std::vector<double> termDensities;
+};
+
 class QueryNode {
    public:
     QueryNode() = default;
@@ -37,6 +45,9 @@ class QueryNode {
     void AddChild(std::unique_ptr<QueryNode>& child);
     virtual QueryNodeType GetNodeType() const = 0;
     virtual std::string GetName() const = 0;
+    // Filter statistics of this subtree (AND: product, OR: 1 - product of complements, NOT:
+    // complement; a term's density is its doc count over the table's docs).
+    virtual void Estimate(const DataTable& dataTable, FilterStats& s) const;
     uint32_t GetChildrenNum() const { return static_cast<uint32_t>(m_children.size()); }
     bool IsLeaf() const {
         QueryNodeType t = GetNodeType();
@@ -87,6 +98,7 @@ class TermNode : public QueryLeafNode {
     bool GetPostOrderExpression(const DataTable& dataTable, std::vector<uint32_t>& postExpr, PostingOperands& addrs,
                                 uint32_t& opNum) const override;
     bool AppendPostings(const DataTable& dataTable, PostingOperands& addrs) const override;
+    void Estimate(const DataTable& dataTable, FilterStats& s) const override;
     QueryNodeType GetNodeType() const override { return QueryNodeType::TermNode; }
     std::string GetName() const override { return "term:" + m_fieldName + ":" + std::to_string(m_token); }
 
@@ -104,6 +116,7 @@ class TermsNode : public QueryLeafNode {
     bool GetPostOrderExpression(const DataTable& dataTable, std::vector<uint32_t>& postExpr, PostingOperands& addrs,
                                 uint32_t& opNum) const override;
     bool AppendPostings(const DataTable& dataTable, PostingOperands& addrs) const override;
+    void Estimate(const DataTable& dataTable, FilterStats& s) const override;
     QueryNodeType GetNodeType() const override { return QueryNodeType::TermsNode; }
     std::string GetName() const override { return "terms:" + m_fieldName; }
 
@@ -128,6 +141,14 @@ class ConjunctionNode : public QueryNode {
     QueryNodeType GetNodeType() const override { return QueryNodeType::ConjunctionNode; }
     std::string GetName() const override { return "conjunction"; }
 };
+
+// The 32 features the floor model reads for one query (k = the keys the main pass selects):
+// [0] log10 selectivity, [1] log10 k, [2] log10 expected passing docs, [3] clip([1] - [2], -9, 3),
+// [4] depth, [5..8] AND / OR / NOT / term counts, [9..12] log10 densities of the rarest terms
+// (ascending, 0 when fewer), [13] query L2 norm, [14] 1 for a match-all filter, the rest 0.
+constexpr uint32_t kFloorFeatures = 32;
+void FloorFeatures(const DataTable& table, QueryNode* tree, uint32_t k, const float* query, uint32_t dim,
+                   float* out);
 
 // The compiled filter of one query: the postfix program, its operands, and whether it matches all.
 struct FilterExpr {

@@ -136,6 +136,12 @@ __global__ void PinballKernel(int rows, const float* __restrict__ y, const float
     dy[b] = (r > 0.0f ? -tau : 1.0f - tau) / static_cast<float>(rows);
 }
 
+__global__ void FloorsKernel(int rows, const float* __restrict__ y, const float* __restrict__ offset,
+                             float* __restrict__ floors) {
+    const int r = blockIdx.x * blockDim.x + threadIdx.x;
+    if (r < rows) floors[r] = y[r] + offset[r];
+}
+
 // Adam (bias-corrected) on FP32 master parameters.
 __global__ void AdamKernel(int64_t n, float* __restrict__ p, const float* __restrict__ g, float* __restrict__ m,
                            float* __restrict__ v, float lr, float beta1, float beta2, float eps, float c1, float c2) {
@@ -286,4 +292,17 @@ extern "C" int frk_mlp_forward(const frk_mlp_params* p, int32_t rows, const floa
                           stream);
     if (rc == FRK_OK) rc = frk_mlp_head_forward(rows, Hd, h2, Hd, p->w3, p->b3, d_y, stream);
     return rc;
+}
+
+extern "C" int frk_mlp_floors(int32_t rows, const float* d_y, const float* d_offset, float* d_floors,
+                              frk_stream_t stream) {
+    if (rows <= 0) return FRK_OK;
+    if (!d_y || !d_offset || !d_floors) {
+        frk::SetError("frk_mlp_floors: null argument");
+        return FRK_ERR_INVALID;
+    }
+    hipLaunchKernelGGL(frk::FloorsKernel, dim3((rows + 255) / 256), dim3(256), 0, static_cast<hipStream_t>(stream),
+                       rows, d_y, d_offset, d_floors);
+    FRK_HIP_CHECK(hipGetLastError());
+    return FRK_OK;
 }

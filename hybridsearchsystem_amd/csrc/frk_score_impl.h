@@ -204,6 +204,11 @@ __device__ __forceinline__ void ComputeGroup(const u32x4* a_lds, int g, const u3
 #endif
 constexpr int kQueueKeys = FRK_SCORE_QUEUE_KEYS;  // keys per wave
 constexpr int kQueueRuns = kQueueKeys / 4;        // runs per wave
+// accumulator rows appended per queue-capacity check: one row adds up to 64 keys (one per lane) and
+// 2 runs (one per lane half), so a group must fit an empty queue
+constexpr int kAppendRows = kQueueKeys / 64 < 16 ? kQueueKeys / 64 : 16;
+static_assert(kAppendRows >= 1 && 16 % kAppendRows == 0, "append groups must tile the 16 rows of a lane");
+static_assert(64 * kAppendRows <= kQueueKeys && 2 * kAppendRows <= kQueueRuns, "an append group must fit the queue");
 
 struct WaveQueue {
     uint64_t* keys;  // [kQueueKeys]
@@ -295,23 +300,24 @@ __device__ __forceinline__ void EpilogueRowBlock(const ScoreArgs& A, const float
         }
         if (!doc_ok) hits = 0;
         if (__ballot(hits != 0) == 0) return;
-        // two halves of 8 rows: count a half's keys and runs first, flush if they would not fit, then
-        // append (so the unrolled append loop itself never flushes)
+        // groups of kAppendRows rows: count a group's keys and runs first, flush if they would not fit,
+        // then append (so the unrolled append loop itself never flushes). A group holds at most
+        // 64 kAppendRows keys and 2 kAppendRows runs, which an empty queue always takes.
 #pragma unroll
-        for (int hb = 0; hb < 2; hb++) {
-            uint64_t bal[8];
+        for (int hb = 0; hb < 16 / kAppendRows; hb++) {
+            uint64_t bal[kAppendRows];
             uint32_t nkeys = 0, nruns = 0;
 #pragma unroll
-            for (int x = 0; x < 8; x++) {
-                bal[x] = __ballot((hits >> (8 * hb + x)) & 1u);
+            for (int x = 0; x < kAppendRows; x++) {
+                bal[x] = __ballot((hits >> (kAppendRows * hb + x)) & 1u);
                 nkeys += static_cast<uint32_t>(__popcll(bal[x]));
                 nruns += (static_cast<uint32_t>(bal[x]) != 0) + (static_cast<uint32_t>(bal[x] >> 32) != 0);
             }
             if (nkeys == 0) continue;
             if (Qu.nk + nkeys > kQueueKeys || Qu.nr + nruns > kQueueRuns) FlushQueue(A, Qu);
 #pragma unroll
-            for (int x = 0; x < 8; x++) {
-                const int r = 8 * hb + x;
+            for (int x = 0; x < kAppendRows; x++) {
+                const int r = kAppendRows * hb + x;
                 if (bal[x] == 0) continue;
                 const uint32_t lo = static_cast<uint32_t>(bal[x]), hi = static_cast<uint32_t>(bal[x] >> 32);
                 const uint32_t below = static_cast<uint32_t>(__popcll(bal[x] & ((1ull << lane) - 1ull)));

@@ -1,10 +1,12 @@
 """Learned score floor: a small MLP that predicts, per query, a lower bound on the k-th best filtered
-score, used by FullRecallSearcher as a per-query floor (``Table.search(floors=...)``) so the fused
-scorer drops docs below it before compaction.
+score, used by FullRecallSearcher as a per-query floor so the fused scorer drops docs below it before
+compaction.
 
 Input: the query vector and 32 filter features (estimated log selectivity under tag independence,
 expected passing docs, k relative to them, AST depth and operator counts, the rarest tags' densities,
-query norm). Network: (dim + 32) -> 256 -> 256 -> 1 with ReLU, trained with a pinball (quantile) loss on
+query norm), computed by the library (frk_floor_features) exactly as the searcher computes them when
+it runs the model on the device for a batch (``Table.search(floor_model=...)``). Network:
+(dim + 32) -> 256 -> 256 -> 1 with ReLU, trained with a pinball (quantile) loss on
 k-th scores the plain FP32 method produces, then shifted down by a margin calibrated on held-out
 queries. Every forward, backward and optimizer step runs in the library's HIP kernels
 (``frk_mlp_*`` in include/frk_capi.h: BF16 MFMA GEMMs with FP32 accumulation); torch only provides
@@ -18,10 +20,8 @@ import ctypes
 import numpy as np
 
 from . import check, lib
-from .search import parse_filter
 
 N_FEAT = 32
-_TERM, _AND, _OR, _NOT = 0, 1, 2, 3
 EPI_NONE, EPI_BIAS_RELU, EPI_RELU_GRAD = 0, 1, 2
 
 
@@ -97,64 +97,22 @@ def adam(param, grad, m, v, lr, step, beta1=0.9, beta2=0.999, eps=1e-8):
 
 
 # ----------------------------------------------------------------------------------------- features
-def _walk(pairs, i, dens):
-    """Pre-order (type, arg) pairs from index i -> (next index, selectivity, depth, counts, term densities)."""
-    typ, arg = int(pairs[i][0]), int(pairs[i][1])
-    if typ == _TERM:
-        d = float(dens[arg]) if 0 <= arg < len(dens) else 0.0
-        return i + 1, d, 1, np.array([0, 0, 0, 1]), [d]
-    j = i + 1
-    sels, depth, counts, terms = [], 0, np.zeros(4, np.int64), []
-    for _ in range(arg):
-        j, s, dp, c, t = _walk(pairs, j, dens)
-        sels.append(s)
-        depth = max(depth, dp)
-        counts += c
-        terms += t
-    if typ == _NOT:
-        sel = 1.0 - sels[0]
-        counts[2] += 1
-    elif typ == _AND:
-        sel = float(np.prod(sels)) if sels else 1.0
-        counts[0] += 1
-    else:
-        sel = 1.0 - float(np.prod([1.0 - s for s in sels])) if sels else 0.0
-        counts[1] += 1
-    return j, sel, depth + 1, counts, terms
-
-
-def filter_features(expr, densities, k, n_docs, query=None):
-    """The 32 filter features of one (filter string, k) against tag densities (index = tag id)."""
-    f = np.zeros(N_FEAT, np.float32)
-    pairs = parse_filter(expr) if expr else np.zeros((0, 2), np.int64)
-    if pairs is None:
-        raise ValueError("filter does not parse: %r" % expr)
-    if len(pairs) == 0:
-        sel, depth, counts, terms = 1.0, 0, np.zeros(4), []
-        f[14] = 1.0
-    else:
-        _, sel, depth, counts, terms = _walk(pairs, 0, densities)
-    sel = min(max(sel, 1e-9), 1.0)
-    passing = max(sel * n_docs, 1.0)
-    f[0] = np.log10(sel)
-    f[1] = np.log10(max(k, 1))
-    f[2] = np.log10(passing)
-    f[3] = np.clip(f[1] - f[2], -9.0, 3.0)
-    f[4] = depth
-    f[5:9] = counts
-    rare = sorted(np.log10(max(d, 1e-9)) for d in terms)[:4]
-    f[9:9 + len(rare)] = rare
-    if query is not None:
-        f[13] = float(np.linalg.norm(query))
-    return f
-
-
-def features(queries, filters, ks, densities, n_docs):
-    return np.stack([filter_features(f, densities, int(k), n_docs, q) for q, f, k in zip(queries, filters, ks)])
+def features(table, queries, filters, ks):
+    """The 32 floor features per (query, filter, k) as the searcher computes them for a batch
+    (frk_floor_features): log10 selectivity under tag independence from the table's per-token doc
+    counts, log10 k, log10 expected passing docs, k relative to them, AST depth and AND/OR/NOT/term
+    counts, the rarest terms' log10 densities, query norm, match-all flag."""
+    return table.floor_features(queries, filters, ks)
 
 
 def expected_passing(feats):
-    return np.power(10.0, feats[:, 2])
+    return np.power(10.0, np.asarray(feats, np.float64)[:, 2])
+
+
+class FloorModelC(ctypes.Structure):
+    """frk_floor_model of include/frk_capi.h."""
+    _fields_ = [("mlp", MlpParams), ("feat_mean", ctypes.c_void_p), ("feat_std", ctypes.c_void_p),
+                ("margin", ctypes.c_float), ("min_passing_ratio", ctypes.c_float)]
 
 
 # ------------------------------------------------------------------------------------------- model
@@ -182,6 +140,7 @@ class FloorModel:
         self.feat_mean = np.zeros(N_FEAT, np.float32)
         self.feat_std = np.ones(N_FEAT, np.float32)
         self.margin = 0.0
+        self.min_passing_ratio = 2.0
         self._ws = {}
 
     # -- buffers
@@ -197,6 +156,16 @@ class FloorModel:
         p = self.p
         return MlpParams(p["w1"].data_ptr(), p["b1"].data_ptr(), p["w2"].data_ptr(), p["b2"].data_ptr(),
                          p["w3"].data_ptr(), p["b3"].data_ptr(), self.in_dim, self.hidden)
+
+    def c_model(self):
+        """The model as the searcher takes it (frk_search_options.floor_model); the returned struct
+        holds references to the host arrays it points at."""
+        mean = np.ascontiguousarray(self.feat_mean, np.float32)
+        std = np.ascontiguousarray(self.feat_std, np.float32)
+        m = FloorModelC(self.params_struct(), mean.ctypes.data, std.ctypes.data, float(self.margin),
+                        float(self.min_passing_ratio))
+        m._keep = (mean, std)
+        return m
 
     def inputs(self, queries, feats):
         """Host rows [query | standardized features] (float32)."""
@@ -290,10 +259,11 @@ class FloorModel:
         return self.margin
 
     def floors(self, queries, feats, ks):
-        """Per-query floors: prediction minus the margin; -inf where fewer than 2k docs are expected
-        to pass (too little to prune, too easy to overshoot)."""
+        """Per-query floors on the host, as the searcher computes them on the device: prediction minus
+        the margin; -inf where fewer than min_passing_ratio * k docs are expected to pass (too little
+        to prune, too easy to overshoot)."""
         fl = self.predict_raw(queries, feats) - self.margin
-        fl[expected_passing(np.asarray(feats)) < 2.0 * np.asarray(ks, np.float64)] = -np.inf
+        fl[expected_passing(feats) < self.min_passing_ratio * np.asarray(ks, np.float64)] = -np.inf
         return fl.astype(np.float32)
 
     # -- persistence (plain arrays, no pickling)

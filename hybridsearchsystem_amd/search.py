@@ -86,6 +86,18 @@ class Table:
             L.frk_builder_free(b)
         return Table(h, vector_field, posting_field, dim)
 
+    def floor_features(self, queries, filters, ks):
+        """The floor model's 32 features per (query, filter, k), as the searcher computes them."""
+        q = np.ascontiguousarray(queries, dtype=np.float32)
+        n, dim = q.shape
+        ks = np.ascontiguousarray(np.broadcast_to(np.asarray(ks, dtype=np.uint32), (n,)))
+        enc = [f.encode() if f is not None else None for f in filters]
+        fl = (ctypes.c_char_p * n)(*enc)
+        out = np.zeros((n, 32), dtype=np.float32)
+        check(lib().frk_floor_features(self.h, self.posting_field.encode(), n, fl, ks.ctypes.data, q.ctypes.data, dim,
+                                       out.ctypes.data), "frk_floor_features")
+        return out
+
     def compile_filter(self, expr):
         cap = 4 * len(expr) + 16
         buf = np.zeros(cap, dtype=np.uint32)
@@ -96,8 +108,9 @@ class Table:
         return buf[:n].copy(), int(npost.value), int(ops.value), bool(ma.value)
 
     def search(self, queries, filters, ks, dtype=FRK_DTYPE_F32, rescore_factor=4, probe=True, probe_step=0,
-               floors=None, cand_cap=0, k_stride=None):
-        """Returns (ids int64 [Q, k_stride], scores float32, counts uint32, passing uint32, stats dict)."""
+               floors=None, cand_cap=0, k_stride=None, floor_model=None):
+        """Returns (ids int64 [Q, k_stride], scores float32, counts uint32, passing uint32, stats dict).
+        floor_model: a learned.FloorModel whose floors the searcher predicts on the device."""
         q = np.ascontiguousarray(queries, dtype=np.float32)
         nq, dim = q.shape
         ks = np.ascontiguousarray(np.broadcast_to(np.asarray(ks, dtype=np.uint32), (nq,)))
@@ -106,7 +119,11 @@ class Table:
         if filters is not None:
             enc = [f.encode() if f is not None else None for f in filters]
             fl = (ctypes.c_char_p * nq)(*enc)
-        opts = SearchOptions(dtype, rescore_factor, int(bool(probe)), probe_step, None, cand_cap)
+        opts = SearchOptions(dtype, rescore_factor, int(bool(probe)), probe_step, None, cand_cap, None)
+        fm = None
+        if floor_model is not None:
+            fm = floor_model.c_model()  # keeps the host arrays it points at alive for the call
+            opts.floor_model = ctypes.cast(ctypes.byref(fm), ctypes.c_void_p)
         if floors is not None:
             fa = np.ascontiguousarray(floors, dtype=np.float32)
             opts.floors = fa.ctypes.data

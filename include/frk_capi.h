@@ -219,6 +219,21 @@ typedef struct {
 int frk_mlp_forward(const frk_mlp_params* p, int32_t rows, const float* d_x, int32_t ldx, float* d_work, float* d_y,
                     frk_stream_t stream);
 
+/* floors[r] = y[r] + offset[r] (an offset of -inf disables the floor of that row). */
+int frk_mlp_floors(int32_t rows, const float* d_y, const float* d_offset, float* d_floors, frk_stream_t stream);
+
+/* A trained floor model as the searcher uses it: input row = [query | (features - mean) / std],
+ * floor = prediction - margin, no floor where fewer than min_passing_ratio * k docs are expected to
+ * pass (k = the keys the main pass selects, k * rescore_factor on low precision). */
+#define FRK_FLOOR_FEATURES 32
+typedef struct {
+    frk_mlp_params mlp;       /* device weights; mlp.in_dim = query dim + FRK_FLOOR_FEATURES */
+    const float* feat_mean;   /* host [FRK_FLOOR_FEATURES] */
+    const float* feat_std;    /* host [FRK_FLOOR_FEATURES] */
+    float margin;
+    float min_passing_ratio;
+} frk_floor_model;
+
 /* ================================================================================================
  * Layer 2: index and searcher handles (C face of NpuRetrieval::DataTable / FullRecallSearcher).
  * ================================================================================================ */
@@ -270,8 +285,9 @@ typedef struct {
     uint32_t rescore_factor;  /* low precision: FP32 rescoring of the top k * m */
     int32_t probe;            /* sampled k-th-best floor (exact pruning) */
     uint32_t probe_step;      /* 0 = automatic */
-    const float* floors;      /* host, per query, or NULL: caller score floors (learned bound) */
+    const float* floors;      /* host, per query, or NULL: caller score floors */
     uint32_t cand_cap;        /* 0 = automatic */
+    const frk_floor_model* floor_model; /* or NULL: learned floors computed on the device per batch */
 } frk_search_options;
 
 typedef struct {
@@ -279,6 +295,12 @@ typedef struct {
     uint64_t candidates;
     uint32_t reruns, overflowed, probe_step, cand_cap;
 } frk_search_stats;
+
+/* The FRK_FLOOR_FEATURES floor-model features of n (filter, k) pairs against the table's posting
+ * field (selectivity under tag independence from the table's per-token doc counts, filter shape,
+ * query norm): out[i * FRK_FLOOR_FEATURES + f]. Host only. */
+int frk_floor_features(const frk_table* t, const char* posting_field, uint32_t n, const char* const* filters,
+                       const uint32_t* ks, const float* queries, uint32_t dim, float* out);
 
 /* FullRecallSearcher::AddQuery x n + BatchSearch (full_recall_searcher.h:30-33) with filter strings
  * (NULL entry or NULL array = match all). Results per query: out_ids[q * k_stride + i] (global doc
