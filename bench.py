@@ -145,6 +145,74 @@ def calibrate_copy(n_bytes=1 << 30):
     print(json.dumps({"calibration": "copy", "bytes": n_bytes}), flush=True)
 
 
+def learned_leg(table, dens, args, queries, filters, ks, res_p, centroids, seed):
+    """Trains the learned score floor on seeded queries (labels: k'-th scores of the plain FP32
+    method, k' = k * m for m in {1, 4}), times BF16 + learned floor + FP32 rescoring on the bench's
+    query stream, and evaluates prune rate, fallback rate and exactness on held-out queries with
+    k in {10, 100, 1000} and filters down to 0.001% selectivity."""
+    import torch
+    from hybridsearchsystem_amd import learned, synth
+    from hybridsearchsystem_amd.search import FRK_DTYPE_BF16, FRK_DTYPE_F32
+    S = synth.SEEDS
+    t0 = time.time()
+    tq = synth.mixture_queries(S["train_queries"], args.learned_train, args.dim, centroids, corpus_seed=seed)
+    tf, _ = synth.random_filters(S["train_filters"], dens, args.learned_train, depth=3, sel=(1e-5, 0.6))
+    tk = synth.draw_ks(S["train_kdraw"], args.learned_train, [10, 100, 1000])
+    log("learned floor: %d training queries drawn in %.1fs" % (args.learned_train, time.time() - t0))
+    model, info = learned.train_for_table(table, args.dim, tq, tf, tk, log=log)
+    log("learned floor: trained (%s)" % info)
+    del tq
+    kw = dict(dtype=FRK_DTYPE_BF16, rescore_factor=4, probe=False, floor_model=model)
+    run_steps(table, queries, filters, ks, args.warmup, args.batch, **kw)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    lat, st, res = run_steps(table, queries, filters, ks, args.steps, args.batch, **kw)
+    torch.cuda.synchronize()
+    wall = time.perf_counter() - t0
+    rec = recall(res, res_p)
+    qps = args.steps * args.batch / wall
+    timed = {"qps": round(qps, 2), "p50_ms": round(float(np.percentile(lat, 50)), 3),
+             "p99_ms": round(float(np.percentile(lat, 99)), 3), "recall_at_k_vs_plain": round(rec, 5),
+             "candidates_per_query": round(float(np.mean([s["candidates"] for s in st])) / args.batch, 1),
+             "reruns_per_batch": round(float(np.mean([s["reruns"] for s in st])), 3),
+             "stages_ms": {k: round(float(np.mean([s[k] for s in st])), 3) for k in
+                           ("ms_filter", "ms_probe", "ms_score", "ms_select", "ms_rescore", "ms_total")}}
+    # held-out evaluation
+    n = args.learned_eval
+    eq = synth.mixture_queries(S["eval_queries"], n, args.dim, centroids, corpus_seed=seed)
+    ef, _ = synth.random_filters(S["eval_filters"], dens, n, depth=3, sel=(1e-5, 0.6))
+    ek = synth.draw_ks(S["eval_kdraw"], n, [10, 100, 1000])
+    ev = {}
+    for name, dtype, m in (("plain_fp32", FRK_DTYPE_F32, 1), ("bf16_rescore4", FRK_DTYPE_BF16, 4)):
+        cand = {"learned": 0}
+        passing = reruns = exact = hits = tot = 0
+        for lo in range(0, n, args.batch):
+            sl = slice(lo, lo + args.batch)
+            truth = table.search(eq[sl], ef[sl], ek[sl], dtype=FRK_DTYPE_F32, probe=False)
+            none = table.search(eq[sl], ef[sl], ek[sl], dtype=dtype, rescore_factor=m, probe=False)
+            got = table.search(eq[sl], ef[sl], ek[sl], dtype=dtype, rescore_factor=m, probe=False, floor_model=model)
+            cand["learned"] += got[4]["candidates"]
+            passing += int(truth[3].sum())
+            reruns += got[4]["reruns"]
+            for q in range(len(truth[2])):
+                a = truth[0][q, :truth[2][q]]
+                b = got[0][q, :got[2][q]]
+                exact += int(np.array_equal(none[0][q, :none[2][q]], b) and
+                             np.array_equal(none[1][q, :none[2][q]], got[1][q, :got[2][q]]))
+                hits += len(set(a.tolist()) & set(b.tolist()))
+                tot += len(a)
+        # prune rate: passing docs the floor dropped before compaction (survivors include any rerun's)
+        ev[name] = {"queries": n, "passing_docs": passing, "survivors_with_learned_floor": cand["learned"],
+                    "prune_rate": round(1.0 - cand["learned"] / max(passing, 1), 5),
+                    "fallback_rate": round(reruns / n, 5), "identical_to_unpruned": round(exact / n, 5),
+                    "recall_at_k_vs_plain_fp32": round(hits / max(tot, 1), 5)}
+        log("learned floor eval %s: %s" % (name, ev[name]))
+    out = {"model": "MLP %d+32 -> 256 -> 256 -> 1, BF16 MFMA training and inference" % args.dim, "train": info,
+           "timed_k100": timed, "heldout_k_10_100_1000": ev}
+    run = ("bf16 + learned floor + fp32 rescore of top k*4", qps, rec, lat, st, wall)
+    return out, run
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -160,6 +228,9 @@ def main():
     ap.add_argument("--cpu-threads", type=int, default=16)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--calibrate-copy", action="store_true", help="run one 1 GiB copy and exit")
+    ap.add_argument("--no-learned", action="store_true", help="skip training and timing the learned floor")
+    ap.add_argument("--learned-train", type=int, default=200_000, help="seeded training queries")
+    ap.add_argument("--learned-eval", type=int, default=8192, help="held-out queries (k in {10,100,1000})")
     ap.add_argument("--traffic-profile", default=os.path.join(ROOT, "profiles", "score_bf16_10Mx256_q128_traffic.md"),
                     help="rocprofv3 counter summary holding the main score kernel's HBM bytes per launch")
     args = ap.parse_args()
@@ -198,10 +269,20 @@ def main():
     rec = recall(res_c, res_p)
     qps_p = args.steps * args.batch / wall_p
     qps_c = args.steps * args.batch / wall_c
-    chosen_ok = rec >= 0.99
-    value = qps_c if chosen_ok else qps_p
-    lat = lat_c if chosen_ok else lat_p
-    st = st_c if chosen_ok else st_p
+    learned_out, lrun = None, None
+    if not args.no_learned:
+        learned_out, lrun = learned_leg(table, dens, args, queries, filters, ks, res_p, centroids, seed)
+    # the fastest low-precision pipeline whose recall@k against plain FP32 is >= 0.99
+    runs = [("bf16 + probe floor + fp32 rescore of top k*4", qps_c, rec, lat_c, st_c, wall_c)]
+    if lrun is not None:
+        runs.append(lrun)
+    ok_runs = [r for r in runs if r[2] >= 0.99]
+    chosen_ok = bool(ok_runs)
+    best = max(ok_runs, key=lambda r: r[1]) if ok_runs else None
+    value = best[1] if chosen_ok else qps_p
+    lat = best[3] if chosen_ok else lat_p
+    st = best[4] if chosen_ok else st_p
+    wall_c = best[5] if chosen_ok else wall_c
 
     # roofline of the dominant kernel: the main fused score pass, timed by HIP events recorded on the
     # searcher's stream around its launch. Algorithmic bytes per launch: the doc stream once, the
@@ -229,13 +310,15 @@ def main():
         "config": {"workload": "10M x 256 filtered top-%d, batch %d" % (args.k, args.batch), "docs": n, "dim": d,
                    "batch": q, "k": args.k, "tags": n_tags, "selectivity": [0.0001, 0.6]},
         "p50_ms": round(float(np.percentile(lat, 50)), 3), "p99_ms": round(float(np.percentile(lat, 99)), 3),
-        "recall_at_k_vs_plain": round(rec, 5),
+        "recall_at_k_vs_plain": round(best[2] if chosen_ok else 1.0, 5),
         "plain_fp32": {"qps": round(qps_p, 2), "p50_ms": round(float(np.percentile(lat_p, 50)), 3),
                        "p99_ms": round(float(np.percentile(lat_p, 99)), 3), "score_kernel_ms": round(ms_score_p, 3),
                        "roofline": {"bound": "mfma", "achieved": round(tf_p, 1), "peak": MFMA_PEAK_TFLOPS["f32"],
                                     "unit": "TFLOP/s", "frac": round(tf_p / MFMA_PEAK_TFLOPS["f32"], 4)}},
-        "chosen": {"qps": round(qps_c, 2), "pipeline": "bf16 + probe floor + fp32 rescore of top k*4",
-                   "speedup_vs_plain": round(qps_c / qps_p, 3)},
+        "chosen": {"qps": round(value, 2), "pipeline": best[0] if chosen_ok else "plain fp32",
+                   "speedup_vs_plain": round(value / qps_p, 3),
+                   "candidates": {r[0]: {"qps": round(r[1], 2), "recall": round(r[2], 5),
+                                         "p99_ms": round(float(np.percentile(r[3], 99)), 3)} for r in runs}},
         "stages_ms": {k: round(float(np.mean([s[k] for s in st])), 3) for k in
                       ("ms_filter", "ms_probe", "ms_score", "ms_select", "ms_rescore", "ms_total")},
         "candidates_per_query": round(float(np.mean([s["candidates"] for s in st])) / q, 1),
@@ -243,6 +326,8 @@ def main():
         "probe_step": int(st[-1]["probe_step"]), "plain_probe_step": int(st_p[-1]["probe_step"]),
         "roofline": roof,
     }
+    if learned_out is not None:
+        out["learned"] = learned_out
     if not args.no_cpu:
         nq = min(args.cpu_queries, args.queries)
         log("cpu baseline: %d queries over %d docs, %d threads" % (nq, head.shape[0], args.cpu_threads))

@@ -298,3 +298,30 @@ def kth_labels(table, queries, filters, ks, batch=128, **search_kw):
             if cnt[i] >= ks[q] and cnt[i] > 0:
                 labels[q] = sc[i, cnt[i] - 1]
     return labels, passing
+
+
+def train_for_table(table, dim, queries, filters, ks, rescore_factors=(1, 4), seed=0, epochs=8, batch=1024, lr=2e-3,
+                    tau=0.02, val_fraction=0.05, fallback=0.005, log=None):
+    """Trains a floor model for `table` on the given seeded queries: each query gets one main-pass key
+    count k' = k * m (m drawn from rescore_factors, so one model serves the plain FP32 path, m = 1,
+    and the low-precision paths), labelled with the k'-th best score of the plain FP32 method. The
+    margin is calibrated on the last val_fraction of the queries for the given fallback rate.
+    Returns (model, info)."""
+    import time
+    rng = np.random.default_rng(seed)
+    m = rng.choice(np.asarray(rescore_factors, np.uint32), size=len(ks))
+    kp = np.minimum(np.asarray(ks, np.uint32) * m, 4096).astype(np.uint32)
+    t0 = time.time()
+    labels, passing = kth_labels(table, queries, filters, kp)
+    t1 = time.time()
+    feats = features(table, queries, filters, kp)
+    n_val = max(1, int(len(kp) * val_fraction))
+    tr, va = slice(0, len(kp) - n_val), slice(len(kp) - n_val, len(kp))
+    model = FloorModel(dim, seed=seed, tau=tau)
+    hist = model.fit(queries[tr], feats[tr], labels[tr], epochs=epochs, batch=batch, lr=lr, seed=seed, log=log)
+    model.calibrate(queries[va], feats[va], labels[va], fallback=fallback)
+    t2 = time.time()
+    info = {"train_queries": int(np.isfinite(labels[tr]).sum()), "val_queries": int(np.isfinite(labels[va]).sum()),
+            "label_s": round(t1 - t0, 2), "train_s": round(t2 - t1, 2), "loss": [round(x, 5) for x in hist],
+            "margin": round(model.margin, 5), "target_fallback": fallback}
+    return model, info

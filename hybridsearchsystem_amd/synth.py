@@ -21,7 +21,8 @@ CONFIGS = {
                ks=(100,), depth=3, tags_per_filter=(1, 4), sel=(0.0001, 0.6), docs_per_segment=131072),
 }
 
-SEEDS = dict(corpus=20261203, queries=1, filters=2, kdraw=3, tags=4)
+SEEDS = dict(corpus=20261203, queries=1, filters=2, kdraw=3, tags=4, train_queries=11, train_filters=12,
+             train_kdraw=13, eval_queries=21, eval_filters=22, eval_kdraw=23)
 
 
 def zipf_densities(n_tags, hi=0.5, lo=1e-5):
@@ -153,8 +154,9 @@ def _tree(rng, tags, depth):
     return ("not", node) if rng.random() < 0.1 else node
 
 
-def random_filters(seed, densities, n, tags_per_filter=(1, 4), depth=3, sel=(0.001, 0.5), max_tries=100000):
+def random_filters(seed, densities, n, tags_per_filter=(1, 4), depth=3, sel=(0.001, 0.5), max_tries=None):
     """n filter strings with their exact selectivities under tag independence."""
+    max_tries = max_tries or max(100000, 200 * n)
     rng = np.random.default_rng(seed)
     n_tags = len(densities)
     out, sels = [], []
