@@ -7,13 +7,13 @@ cp = ctypes.c_char_p
 
 class SearchOptions(ctypes.Structure):
     _fields_ = [("dtype", u32), ("rescore_factor", u32), ("probe", ctypes.c_int32), ("probe_step", u32),
-                ("floors", vp), ("cand_cap", u32), ("floor_model", vp)]
+                ("floors", vp), ("cand_cap", u32), ("floor_model", vp), ("graph", u32)]
 
 
 class SearchStats(ctypes.Structure):
     _fields_ = [("ms_filter", f32), ("ms_probe", f32), ("ms_score", f32), ("ms_select", f32), ("ms_rescore", f32),
                 ("ms_total", f32), ("candidates", u64), ("reruns", u32), ("overflowed", u32), ("probe_step", u32),
-                ("cand_cap", u32)]
+                ("cand_cap", u32), ("graph_replayed", u32)]
 
 
 def bind(L):
@@ -62,6 +62,8 @@ def bind(L):
     L.frk_compile_filter.restype = i32
     L.frk_compile_filter.argtypes = [vp, cp, cp, vp, ctypes.c_int32, ctypes.POINTER(u32), ctypes.POINTER(u32),
                                      ctypes.POINTER(ctypes.c_int32)]
+    L.frk_fill_u32.restype = i32
+    L.frk_fill_u32.argtypes = [vp, u64, u32, vp]
     L.frk_floor_features.restype = i32
     L.frk_floor_features.argtypes = [vp, cp, u32, vp, vp, vp, u32, vp]
     L.frk_search.restype = i32

@@ -10,6 +10,7 @@
 #include "../../../include/frk_capi.h"
 #include "data_table.h"
 #include "full_recall_searcher.h"
+#include "search_server.h"
 #include "log.h"
 #include "query_node.h"
 
@@ -21,6 +22,10 @@ struct frk_table {
 
 struct frk_builder {
     std::unique_ptr<TableBuilder> b;
+};
+
+struct frk_server {
+    std::unique_ptr<SearchServer> s;
 };
 
 namespace {
@@ -230,6 +235,7 @@ int frk_search(const frk_table* t, const char* vector_field, const char* posting
         o.floors = opts->floors;
         o.candCap = opts->cand_cap;
         o.floorModel = opts->floor_model;
+        o.graph = opts->graph != 0;
         searcher.SetOptions(o);
     }
     std::vector<AggrDeviceResult> dev;
@@ -257,9 +263,94 @@ int frk_search(const frk_table* t, const char* vector_field, const char* posting
         stats->reruns = s.reruns;
         stats->overflowed = s.overflowed;
         stats->probe_step = s.probeStep;
+        stats->graph_replayed = s.graphReplayed ? 1u : 0u;
         stats->cand_cap = s.cap;
     }
     return FRK_OK;
 }
+
+int frk_server_create(const frk_table* t, const char* vector_field, const char* posting_field,
+                      const frk_search_options* opts, const frk_server_config* cfg, frk_server** out) {
+    if (!t || !vector_field || !posting_field || !out) return Fail(FRK_ERR_INVALID, "frk_server_create: null argument");
+    ServerConfig c;
+    c.vectorField = vector_field;
+    c.postingField = posting_field;
+    if (opts) {
+        c.options.dtype = opts->dtype;
+        c.options.rescoreFactor = opts->rescore_factor;
+        c.options.probe = opts->probe != 0;
+        c.options.probeStep = opts->probe_step;
+        c.options.floors = nullptr;  // per-request floors have no meaning across batches
+        c.options.candCap = opts->cand_cap;
+        c.options.floorModel = opts->floor_model;
+        c.options.graph = opts->graph != 0;
+    }
+    if (cfg) {
+        if (cfg->max_batch_queries) c.maxBatchQueries = cfg->max_batch_queries;
+        if (cfg->workers) c.workers = cfg->workers;
+        c.hbmBudgetBytes = cfg->hbm_budget_bytes;
+    }
+    VectorFieldData* vf = nullptr;
+    if (!t->table->GetVectorFieldData(c.vectorField, vf) || !vf)
+        return Fail(FRK_ERR_INVALID, "frk_server_create: no such vector field");
+    auto* srv = new frk_server;
+    srv->s = std::make_unique<SearchServer>(t->table, t->table->GetDeviceId(), c);
+    *out = srv;
+    return FRK_OK;
+}
+
+void frk_server_free(frk_server* s) { delete s; }
+
+int frk_server_submit(frk_server* s, uint32_t n_queries, const float* queries, uint32_t dim,
+                      const char* const* filters, const uint32_t* ks, uint64_t* ticket) {
+    if (!s || !queries || !ks || !ticket) return Fail(FRK_ERR_INVALID, "frk_server_submit: null argument");
+    std::vector<std::string> f;
+    if (filters)
+        for (uint32_t i = 0; i < n_queries; i++) f.emplace_back(filters[i] ? filters[i] : "");
+    std::vector<uint32_t> k(ks, ks + n_queries);
+    *ticket = s->s->Submit(n_queries, queries, dim, f, k);
+    return *ticket ? FRK_OK : Fail(FRK_ERR_INVALID, "frk_server_submit: request rejected");
+}
+
+int frk_server_wait(frk_server* s, uint64_t ticket, uint32_t k_stride, int64_t* out_ids, float* out_scores,
+                    uint32_t* out_counts, uint32_t* out_passing, double* times, uint32_t* batch_queries) {
+    if (!s || !out_ids || !out_counts) return Fail(FRK_ERR_INVALID, "frk_server_wait: null argument");
+    RequestResult r;
+    const bool ok = s->s->Wait(ticket, r);
+    if (times) {
+        times[0] = r.submitted;
+        times[1] = r.started;
+        times[2] = r.finished;
+    }
+    if (batch_queries) *batch_queries = r.batchQueries;
+    if (!ok) return Fail(FRK_ERR_INVALID, "frk_server_wait: request failed");
+    for (size_t q = 0; q < r.counts.size(); q++) {
+        const uint32_t n = std::min<uint32_t>(r.counts[q], k_stride);
+        for (uint32_t i = 0; i < k_stride; i++) {
+            out_ids[q * k_stride + i] = i < n ? r.ids[q][i] : -1;
+            if (out_scores) out_scores[q * k_stride + i] = i < n ? r.scores[q][i] : 0.0f;
+        }
+        out_counts[q] = n;
+        if (out_passing) out_passing[q] = r.passing[q];
+    }
+    return FRK_OK;
+}
+
+int frk_server_stats_get(const frk_server* s, frk_server_stats* out) {
+    if (!s || !out) return Fail(FRK_ERR_INVALID, "frk_server_stats_get: null argument");
+    const ServerStats st = s->s->Stats();
+    out->requests = st.requests;
+    out->queries = st.queries;
+    out->batches = st.batches;
+    out->graph_replays = st.graphReplays;
+    out->reruns = st.reruns;
+    out->device_ms = st.deviceMs;
+    out->doc_bytes_streamed = st.docBytesStreamed;
+    return FRK_OK;
+}
+
+double frk_server_now(const frk_server* s) { return s ? s->s->Now() : 0.0; }
+
+uint64_t frk_server_batch_bytes(const frk_server* s, uint32_t n_queries) { return s ? s->s->BatchBytes(n_queries) : 0; }
 
 }  // extern "C"

@@ -52,10 +52,33 @@ class SearchWorkspace {
     }
     ~SearchWorkspace() {
         (void)hipSetDevice(m_device);
-        for (Buf* b : {&batch, &bitmaps, &bmasks, &counters, &cand, &probe, &lp, &rescored, &out, &hbatch, &hout})
+        for (auto& g : m_graphs) (void)hipGraphExecDestroy(g.exec);
+        for (Buf* b : {&batch, &bitmaps, &bmasks, &counters, &cand, &probe, &lp, &rescored, &out, &mlp, &hbatch, &hout})
             if (b->p) b->pinned ? (void)hipHostFree(b->p) : (void)hipFree(b->p);
         for (auto& e : ev) (void)hipEventDestroy(e);
         (void)hipStreamDestroy(stream);
+    }
+
+    // Instantiated pipeline graphs of this workspace, keyed by every scalar and address the captured
+    // sequence bakes in; a key is captured on its second batch (the first runs eagerly).
+    hipGraphExec_t FindGraph(const std::vector<uint64_t>& key) const {
+        for (const auto& g : m_graphs)
+            if (g.key == key) return g.exec;
+        return nullptr;
+    }
+    bool SeenBefore(const std::vector<uint64_t>& key) const {
+        return std::find(m_seen.begin(), m_seen.end(), key) != m_seen.end();
+    }
+    void MarkSeen(const std::vector<uint64_t>& key) {
+        if (m_seen.size() >= 32) m_seen.erase(m_seen.begin());
+        m_seen.push_back(key);
+    }
+    void AddGraph(const std::vector<uint64_t>& key, hipGraphExec_t exec) {
+        if (m_graphs.size() >= 16) {
+            (void)hipGraphExecDestroy(m_graphs.front().exec);
+            m_graphs.erase(m_graphs.begin());
+        }
+        m_graphs.push_back({key, exec});
     }
 
     static std::unique_ptr<SearchWorkspace> Acquire(int32_t device) {
@@ -88,6 +111,12 @@ class SearchWorkspace {
         static auto* p = new std::vector<std::unique_ptr<SearchWorkspace>>();  // lives until exit
         return *p;
     }
+    struct GraphEntry {
+        std::vector<uint64_t> key;
+        hipGraphExec_t exec;
+    };
+    std::vector<GraphEntry> m_graphs;
+    std::vector<std::vector<uint64_t>> m_seen;
     int32_t m_device;
 };
 
@@ -240,17 +269,11 @@ bool FullRecallSearcher::RunDevice(const VectorFieldData& vf, std::vector<AggrDe
         return false;
     }
 
-    // ---- pack the batch: descriptors | expr | table | queries (per store dim) | k | k*m | floors
+    // ---- pack the batch. Fixed-size regions first (descriptors | queries per store dim | k | k*m |
+    // floors | model inputs | floor offsets: their offsets depend only on the batch shape), then the
+    // filter programs and operand table in power-of-two capacities, so a captured pipeline graph sees
+    // the same device addresses for every batch of its shape.
     const uint32_t dpMain = store->dim, dpF32 = f32->dim;
-    const size_t oDesc = 0;
-    const size_t oExpr = oDesc + Pad<frk_filter_query>(Q);
-    const size_t oTab = oExpr + Pad<uint32_t>(expr.size() + 1);
-    const size_t oQMain = oTab + Pad<uint64_t>(table.size() + 1);
-    const size_t oQF32 = oQMain + Pad<float>(static_cast<size_t>(Q) * dpMain);
-    const size_t oK = oQF32 + (lowp ? Pad<float>(static_cast<size_t>(Q) * dpF32) : 0);
-    const size_t oKm = oK + Pad<uint32_t>(Q);
-    const size_t oFloor = oKm + Pad<uint32_t>(Q);
-    // learned floors: model input rows [query | standardized features] and per-query offsets
     const frk_floor_model* model = m_options.floorModel;
     const bool learned = model != nullptr && m_options.floors == nullptr;
     const uint32_t inDim = learned ? static_cast<uint32_t>(model->mlp.in_dim) : 0;
@@ -260,9 +283,23 @@ bool FullRecallSearcher::RunDevice(const VectorFieldData& vf, std::vector<AggrDe
                                              << kFloorFeatures << " features");
         return false;
     }
+    auto pow2 = [](size_t n) {
+        size_t c = 256;
+        while (c < n) c <<= 1;
+        return c;
+    };
+    const size_t exprCap = pow2(expr.size() + 1), tableCap = pow2(table.size() + 1);
+    const size_t oDesc = 0;
+    const size_t oQMain = oDesc + Pad<frk_filter_query>(Q);
+    const size_t oQF32 = oQMain + Pad<float>(static_cast<size_t>(Q) * dpMain);
+    const size_t oK = oQF32 + (lowp ? Pad<float>(static_cast<size_t>(Q) * dpF32) : 0);
+    const size_t oKm = oK + Pad<uint32_t>(Q);
+    const size_t oFloor = oKm + Pad<uint32_t>(Q);
     const size_t oX = oFloor + Pad<float>(Q);
     const size_t oOff = oX + (learned ? Pad<float>(static_cast<size_t>(Q) * inDim) : 0);
-    const size_t batchBytes = oOff + (learned ? Pad<float>(Q) : 0);
+    const size_t oExpr = oOff + (learned ? Pad<float>(Q) : 0);
+    const size_t oTab = oExpr + Pad<uint32_t>(exprCap);
+    const size_t batchBytes = oTab + Pad<uint64_t>(tableCap);
     std::unique_ptr<SearchWorkspace> ws = SearchWorkspace::Acquire(m_deviceId);
     struct Releaser {
         std::unique_ptr<SearchWorkspace>& w;
@@ -352,73 +389,130 @@ bool FullRecallSearcher::RunDevice(const VectorFieldData& vf, std::vector<AggrDe
         return false;
     };
 
-    // ---- the pipeline
-    (void)hipEventRecord(ws->ev[0], s);
-    if (hipMemcpyAsync(db, hb, batchBytes, hipMemcpyHostToDevice, s) != hipSuccess) return fail("batch H2D");
-    if (hipMemsetAsync(dCnt, 0, sizeof(uint32_t) * nCounters, s) != hipSuccess) return fail("counter reset");
+    // ---- the pipeline: one stream-ordered sequence, launched eagerly or replayed from a graph
     const float* dQMain = reinterpret_cast<const float*>(db + oQMain);
     const float* dQF32 = lowp ? reinterpret_cast<const float*>(db + oQF32) : dQMain;
     const uint32_t* dK = reinterpret_cast<const uint32_t*>(db + oK);
     const uint32_t* dKm = reinterpret_cast<const uint32_t*>(db + oKm);
     uint64_t* dMask = ws->bitmaps.As<uint64_t>();
-    if (frk_filter_eval(reinterpret_cast<const frk_filter_query*>(db + oDesc), Q,
-                        reinterpret_cast<const uint32_t*>(db + oExpr), reinterpret_cast<const uint64_t*>(db + oTab),
-                        segs, dnps, nDocs, dMask, maskStride, dCnt + cFilter, dCnt + cStatus, maxPostings, s) != 0)
-        return fail("filter");
     uint32_t* dBMask = ws->bmasks.As<uint32_t>();
-    if (frk_masks_to_blocks(dMask, maskStride, Q, nDocs, dBMask, qStride, s) != 0) return fail("mask transpose");
-    (void)hipEventRecord(ws->ev[1], s);
-    const float* dFloor = callerFloor ? reinterpret_cast<const float*>(db + oFloor) : nullptr;
-    if (useProbe) {
-        // the k-th (k*m on low precision) largest block maximum of the sample is the floor
-        if (frk_score_blockmax(store, dQMain, Q, dBMask, qStride, 0, probeStep, probeBlocks, ws->probe.As<float>(),
-                               static_cast<uint32_t>(probeBlocks), s) != 0)
-            return fail("probe score");
-        if (frk_kth_largest(ws->probe.As<float>(), static_cast<uint32_t>(probeBlocks),
-                            static_cast<uint32_t>(probeBlocks), Q, dKm, 0, dKth, s) != 0)
-            return fail("probe select");
-        dFloor = dKth;
-    }
-    if (learned) {
-        // the floor model on this stream: prediction - margin, or -inf where no floor applies
-        float* work = ws->mlp.As<float>();
-        float* y = work + 4ull * Q * static_cast<uint32_t>(model->mlp.hidden);
-        float* fl = y + Q;
-        if (frk_mlp_forward(&model->mlp, static_cast<int32_t>(Q), reinterpret_cast<const float*>(db + oX),
-                            static_cast<int32_t>(inDim), work, y, s) != 0 ||
-            frk_mlp_floors(static_cast<int32_t>(Q), y, reinterpret_cast<const float*>(db + oOff), fl, s) != 0)
-            return fail("floor model");
-        dFloor = fl;
-    }
-    (void)hipEventRecord(ws->ev[2], s);
-    if (frk_score_select(store, dQMain, Q, dBMask, qStride, dFloor, 0, 1, nBlocks, ws->cand.As<uint64_t>(), cap,
-                         dCnt + cCand, s) != 0)
-        return fail("score");
-    (void)hipEventRecord(ws->ev[3], s);
-    if (!lowp) {
-        if (frk_select_topk(ws->cand.As<uint64_t>(), cap, dCnt + cCand, Q, dK, 0, ws->out.As<uint64_t>(), kmax,
-                            dCnt + cOut, nullptr, s) != 0)
-            return fail("select");
-        (void)hipEventRecord(ws->ev[4], s);
-        (void)hipEventRecord(ws->ev[5], s);
-    } else {
-        if (frk_select_topk(ws->cand.As<uint64_t>(), cap, dCnt + cCand, Q, dKm, 0, ws->lp.As<uint64_t>(), kSel,
-                            dCnt + cLp, nullptr, s) != 0)
-            return fail("select k*m");
-        (void)hipEventRecord(ws->ev[4], s);
-        if (frk_rescore(f32, dQF32, Q, ws->lp.As<uint64_t>(), kSel, dCnt + cLp, ws->rescored.As<uint64_t>(), s) != 0)
-            return fail("rescore");
-        if (frk_select_topk(ws->rescored.As<uint64_t>(), kSel, dCnt + cLp, Q, dK, 0, ws->out.As<uint64_t>(), kmax,
-                            dCnt + cOut, nullptr, s) != 0)
-            return fail("select after rescore");
-        (void)hipEventRecord(ws->ev[5], s);
-    }
     uint8_t* ho = ws->hout.As<uint8_t>();
-    if (hipMemcpyAsync(ho, ws->out.p, sizeof(uint64_t) * Q * kmax, hipMemcpyDeviceToHost, s) != hipSuccess ||
-        hipMemcpyAsync(ho + sizeof(uint64_t) * Q * kmax, dCnt, sizeof(uint32_t) * nCounters, hipMemcpyDeviceToHost, s) !=
-            hipSuccess)
-        return fail("result D2H");
-    (void)hipEventRecord(ws->ev[6], s);
+    // the filter launch only distinguishes up to 4, up to 8 and more operands (its LDS slot sizing)
+    const uint32_t mpLaunch = maxPostings <= 4 ? 4 : maxPostings <= 8 ? 8 : 16;
+    auto enqueue = [&](bool events) -> const char* {
+        auto rec = [&](int i) {
+            if (events) (void)hipEventRecord(ws->ev[i], s);
+        };
+        rec(0);
+        if (hipMemcpyAsync(db, hb, batchBytes, hipMemcpyHostToDevice, s) != hipSuccess) return "batch H2D";
+        if (frk_fill_u32(dCnt, nCounters, 0, s) != 0) return "counter reset";
+        if (frk_filter_eval(reinterpret_cast<const frk_filter_query*>(db + oDesc), Q,
+                            reinterpret_cast<const uint32_t*>(db + oExpr), reinterpret_cast<const uint64_t*>(db + oTab),
+                            segs, dnps, nDocs, dMask, maskStride, dCnt + cFilter, dCnt + cStatus, mpLaunch, s) != 0)
+            return "filter";
+        if (frk_masks_to_blocks(dMask, maskStride, Q, nDocs, dBMask, qStride, s) != 0) return "mask transpose";
+        rec(1);
+        const float* dFloor = callerFloor ? reinterpret_cast<const float*>(db + oFloor) : nullptr;
+        if (useProbe) {
+            // the k-th (k*m on low precision) largest block maximum of the sample is the floor
+            if (frk_score_blockmax(store, dQMain, Q, dBMask, qStride, 0, probeStep, probeBlocks, ws->probe.As<float>(),
+                                   static_cast<uint32_t>(probeBlocks), s) != 0)
+                return "probe score";
+            if (frk_kth_largest(ws->probe.As<float>(), static_cast<uint32_t>(probeBlocks),
+                                static_cast<uint32_t>(probeBlocks), Q, dKm, 0, dKth, s) != 0)
+                return "probe select";
+            dFloor = dKth;
+        }
+        if (learned) {
+            // the floor model on this stream: prediction - margin, or -inf where no floor applies
+            float* work = ws->mlp.As<float>();
+            float* y = work + 4ull * Q * static_cast<uint32_t>(model->mlp.hidden);
+            float* fl = y + Q;
+            if (frk_mlp_forward(&model->mlp, static_cast<int32_t>(Q), reinterpret_cast<const float*>(db + oX),
+                                static_cast<int32_t>(inDim), work, y, s) != 0 ||
+                frk_mlp_floors(static_cast<int32_t>(Q), y, reinterpret_cast<const float*>(db + oOff), fl, s) != 0)
+                return "floor model";
+            dFloor = fl;
+        }
+        rec(2);
+        if (frk_score_select(store, dQMain, Q, dBMask, qStride, dFloor, 0, 1, nBlocks, ws->cand.As<uint64_t>(), cap,
+                             dCnt + cCand, s) != 0)
+            return "score";
+        rec(3);
+        if (!lowp) {
+            if (frk_select_topk(ws->cand.As<uint64_t>(), cap, dCnt + cCand, Q, dK, 0, ws->out.As<uint64_t>(), kmax,
+                                dCnt + cOut, nullptr, s) != 0)
+                return "select";
+            rec(4);
+            rec(5);
+        } else {
+            if (frk_select_topk(ws->cand.As<uint64_t>(), cap, dCnt + cCand, Q, dKm, 0, ws->lp.As<uint64_t>(), kSel,
+                                dCnt + cLp, nullptr, s) != 0)
+                return "select k*m";
+            rec(4);
+            if (frk_rescore(f32, dQF32, Q, ws->lp.As<uint64_t>(), kSel, dCnt + cLp, ws->rescored.As<uint64_t>(), s) != 0)
+                return "rescore";
+            if (frk_select_topk(ws->rescored.As<uint64_t>(), kSel, dCnt + cLp, Q, dK, 0, ws->out.As<uint64_t>(), kmax,
+                                dCnt + cOut, nullptr, s) != 0)
+                return "select after rescore";
+            rec(5);
+        }
+        if (hipMemcpyAsync(ho, ws->out.p, sizeof(uint64_t) * Q * kmax, hipMemcpyDeviceToHost, s) != hipSuccess ||
+            hipMemcpyAsync(ho + sizeof(uint64_t) * Q * kmax, dCnt, sizeof(uint32_t) * nCounters, hipMemcpyDeviceToHost,
+                           s) != hipSuccess)
+            return "result D2H";
+        rec(6);
+        return nullptr;
+    };
+    bool replayed = false;
+    if (!m_options.graph) {
+        if (const char* what = enqueue(true)) return fail(what);
+    } else {
+        // every scalar and address the sequence bakes into its launches
+        const uint64_t key[] = {Q, kmax, kSel, m, dtype, reinterpret_cast<uint64_t>(store->tiles),
+                                reinterpret_cast<uint64_t>(f32->tiles), nDocs, segs, dnps, useProbe, probeStep,
+                                probeBlocks, learned, cap, mpLaunch, batchBytes, oExpr, oTab, callerFloor, qStride,
+                                reinterpret_cast<uint64_t>(ws->batch.p), reinterpret_cast<uint64_t>(ws->hbatch.p),
+                                reinterpret_cast<uint64_t>(ws->bitmaps.p), reinterpret_cast<uint64_t>(ws->bmasks.p),
+                                reinterpret_cast<uint64_t>(ws->counters.p), reinterpret_cast<uint64_t>(ws->cand.p),
+                                reinterpret_cast<uint64_t>(ws->probe.p), reinterpret_cast<uint64_t>(ws->lp.p),
+                                reinterpret_cast<uint64_t>(ws->rescored.p), reinterpret_cast<uint64_t>(ws->out.p),
+                                reinterpret_cast<uint64_t>(ws->mlp.p), reinterpret_cast<uint64_t>(ws->hout.p),
+                                learned ? reinterpret_cast<uint64_t>(model->mlp.w1) : 0,
+                                learned ? reinterpret_cast<uint64_t>(model->mlp.w2) : 0,
+                                learned ? reinterpret_cast<uint64_t>(model->mlp.w3) : 0,
+                                learned ? reinterpret_cast<uint64_t>(model->mlp.b1) : 0,
+                                learned ? reinterpret_cast<uint64_t>(model->mlp.b2) : 0,
+                                learned ? reinterpret_cast<uint64_t>(model->mlp.b3) : 0,
+                                learned ? static_cast<uint64_t>(model->mlp.hidden) : 0};
+        const std::vector<uint64_t> k(std::begin(key), std::end(key));
+        hipGraphExec_t exec = ws->FindGraph(k);
+        if (!exec && ws->SeenBefore(k)) {
+            // second batch of this shape: capture the sequence (the first ran eagerly, so one-time
+            // kernel setup is done) and keep the instantiated graph
+            hipGraph_t g = nullptr;
+            if (hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal) != hipSuccess) return fail("graph capture");
+            const char* what = enqueue(false);
+            const hipError_t e = hipStreamEndCapture(s, &g);
+            if (what || e != hipSuccess) {
+                if (g) (void)hipGraphDestroy(g);
+                return fail(what ? what : "graph capture");
+            }
+            const hipError_t ie = hipGraphInstantiate(&exec, g, nullptr, nullptr, 0);
+            (void)hipGraphDestroy(g);
+            if (ie != hipSuccess) return fail("graph instantiate");
+            ws->AddGraph(k, exec);
+        }
+        if (exec) {
+            (void)hipEventRecord(ws->ev[0], s);
+            if (hipGraphLaunch(exec, s) != hipSuccess) return fail("graph launch");
+            (void)hipEventRecord(ws->ev[6], s);
+            replayed = true;
+        } else {
+            ws->MarkSeen(k);
+            if (const char* what = enqueue(true)) return fail(what);
+        }
+    }
     if (hipStreamSynchronize(s) != hipSuccess) return fail("batch synchronize");
 
     const uint64_t* hKeys = reinterpret_cast<const uint64_t*>(ho);
@@ -432,12 +526,15 @@ bool FullRecallSearcher::RunDevice(const VectorFieldData& vf, std::vector<AggrDe
         (void)hipEventElapsedTime(&t, ws->ev[a], ws->ev[b]);
         return t;
     };
-    m_stats.ms_filter = ms(0, 1);
-    m_stats.ms_probe = ms(1, 2);
-    m_stats.ms_score = ms(2, 3);
-    m_stats.ms_select = ms(3, 4);
-    m_stats.ms_rescore = ms(4, 5);
+    if (!replayed) {
+        m_stats.ms_filter = ms(0, 1);
+        m_stats.ms_probe = ms(1, 2);
+        m_stats.ms_score = ms(2, 3);
+        m_stats.ms_select = ms(3, 4);
+        m_stats.ms_rescore = ms(4, 5);
+    }
     m_stats.ms_total = ms(0, 6);
+    m_stats.graphReplayed = replayed;
     m_stats.cap = cap;
 
     // ---- exactness checks: overflowed candidate lists, and a caller or learned floor that cut below

@@ -56,6 +56,7 @@ struct SearchOptions {
     uint32_t candCap = 0;            // candidate capacity per query; 0 = automatic
     const frk_floor_model* floorModel = nullptr;  // learned floors, predicted on the device per batch
                                                   // (same rerun rule as caller floors)
+    bool graph = false;  // replay the batch's device sequence from a captured hipGraph per batch shape
 };
 
 struct BatchStats {
@@ -65,6 +66,7 @@ struct BatchStats {
     uint32_t overflowed = 0;
     uint32_t probeStep = 0;
     uint32_t cap = 0;
+    bool graphReplayed = false;  // the device sequence ran as a graph replay (stage times then 0)
 };
 
 class SearchWorkspace;

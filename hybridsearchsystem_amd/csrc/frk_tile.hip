@@ -83,3 +83,26 @@ extern "C" int frk_tile_rows_device(const float* d_rows, uint64_t n_rows, uint32
     FRK_HIP_CHECK(hipGetLastError());
     return FRK_OK;
 }
+
+namespace frk {
+namespace {
+__global__ void FillU32Kernel(uint32_t* __restrict__ d, uint64_t n, uint32_t value) {
+    for (uint64_t i = static_cast<uint64_t>(blockIdx.x) * blockDim.x + threadIdx.x; i < n;
+         i += static_cast<uint64_t>(gridDim.x) * blockDim.x)
+        d[i] = value;
+}
+}  // namespace
+}  // namespace frk
+
+extern "C" int frk_fill_u32(uint32_t* d, uint64_t n, uint32_t value, frk_stream_t stream) {
+    if (n == 0) return FRK_OK;
+    if (!d) {
+        frk::SetError("frk_fill_u32: null pointer");
+        return FRK_ERR_INVALID;
+    }
+    const uint64_t blocks = (n + 255) / 256;
+    hipLaunchKernelGGL(frk::FillU32Kernel, dim3(static_cast<uint32_t>(blocks < 1024 ? blocks : 1024)), dim3(256), 0,
+                       static_cast<hipStream_t>(stream), d, n, value);
+    FRK_HIP_CHECK(hipGetLastError());
+    return FRK_OK;
+}

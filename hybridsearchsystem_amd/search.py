@@ -108,9 +108,10 @@ class Table:
         return buf[:n].copy(), int(npost.value), int(ops.value), bool(ma.value)
 
     def search(self, queries, filters, ks, dtype=FRK_DTYPE_F32, rescore_factor=4, probe=True, probe_step=0,
-               floors=None, cand_cap=0, k_stride=None, floor_model=None):
+               floors=None, cand_cap=0, k_stride=None, floor_model=None, graph=False):
         """Returns (ids int64 [Q, k_stride], scores float32, counts uint32, passing uint32, stats dict).
-        floor_model: a learned.FloorModel whose floors the searcher predicts on the device."""
+        floor_model: a learned.FloorModel whose floors the searcher predicts on the device.
+        graph: replay the device sequence from a hipGraph captured per batch shape."""
         q = np.ascontiguousarray(queries, dtype=np.float32)
         nq, dim = q.shape
         ks = np.ascontiguousarray(np.broadcast_to(np.asarray(ks, dtype=np.uint32), (nq,)))
@@ -119,7 +120,7 @@ class Table:
         if filters is not None:
             enc = [f.encode() if f is not None else None for f in filters]
             fl = (ctypes.c_char_p * nq)(*enc)
-        opts = SearchOptions(dtype, rescore_factor, int(bool(probe)), probe_step, None, cand_cap, None)
+        opts = SearchOptions(dtype, rescore_factor, int(bool(probe)), probe_step, None, cand_cap, None, int(bool(graph)))
         fm = None
         if floor_model is not None:
             fm = floor_model.c_model()  # keeps the host arrays it points at alive for the call

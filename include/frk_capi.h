@@ -100,6 +100,9 @@ typedef struct {
 
 uint32_t frk_chunk_dims(uint32_t dtype);
 
+/* d[i] = value for i < n (a kernel, so it may sit inside a captured graph). */
+int frk_fill_u32(uint32_t* d, uint64_t n, uint32_t value, frk_stream_t stream);
+
 /* Host-side staging of a store: row-major FP32 [n_docs][dim] -> tiles (host memory of
  * frk_tiles_bytes() bytes; dims >= dim are zero-filled up to dim_pad). FP8 also writes the per-doc
  * scale max|v| / 448 and stores v / scale. No device work; n_threads 0 = all cores. */
@@ -288,12 +291,14 @@ typedef struct {
     const float* floors;      /* host, per query, or NULL: caller score floors */
     uint32_t cand_cap;        /* 0 = automatic */
     const frk_floor_model* floor_model; /* or NULL: learned floors computed on the device per batch */
+    uint32_t graph;           /* nonzero: replay the device sequence from a captured hipGraph per
+                                 batch shape (captured on the shape's second batch) */
 } frk_search_options;
 
 typedef struct {
-    float ms_filter, ms_probe, ms_score, ms_select, ms_rescore, ms_total;
+    float ms_filter, ms_probe, ms_score, ms_select, ms_rescore, ms_total; /* stage times 0 on a replay */
     uint64_t candidates;
-    uint32_t reruns, overflowed, probe_step, cand_cap;
+    uint32_t reruns, overflowed, probe_step, cand_cap, graph_replayed;
 } frk_search_stats;
 
 /* The FRK_FLOOR_FEATURES floor-model features of n (filter, k) pairs against the table's posting
@@ -310,6 +315,43 @@ int frk_search(const frk_table* t, const char* vector_field, const char* posting
                const float* queries, uint32_t dim, const char* const* filters, const uint32_t* ks,
                const frk_search_options* opts, uint32_t k_stride, int64_t* out_ids, float* out_scores,
                uint32_t* out_counts, uint32_t* out_passing, frk_search_stats* stats);
+
+/* ------------------------------------------------------------------------------------------------
+ * Serving: a continuous-batching request server over the searcher (extends the search driver's
+ * serial --target_qps mode, harness/run/schedule_serial.cpp:23-51). Requests of any number of queries
+ * (each with its own filter and k) are submitted at any time from any thread; a scheduler packs the
+ * oldest pending requests into device batches of at most max_batch_queries queries, starting one as
+ * soon as one of `workers` slots is free and its working set fits the HBM budget left by the batches
+ * in flight, so batches overlap on the device. The index stays resident; with opts->graph each
+ * batch shape's device sequence is replayed from a captured hipGraph. opts->floor_model (if any)
+ * must stay valid while the server lives. Times are seconds on the server's clock.
+ * ------------------------------------------------------------------------------------------------ */
+typedef struct frk_server frk_server;
+typedef struct {
+    uint32_t max_batch_queries; /* 0 = 256 */
+    uint32_t workers;           /* batches in flight; 0 = 2 */
+    uint64_t hbm_budget_bytes;  /* 0 = unlimited */
+} frk_server_config;
+
+typedef struct {
+    uint64_t requests, queries, batches, graph_replays, reruns;
+    double device_ms;            /* sum over batches of the device time of each */
+    uint64_t doc_bytes_streamed; /* doc-store bytes read by the main score passes */
+} frk_server_stats;
+
+int frk_server_create(const frk_table* t, const char* vector_field, const char* posting_field,
+                      const frk_search_options* opts, const frk_server_config* cfg, frk_server** out);
+void frk_server_free(frk_server* s); /* finishes every submitted request first */
+int frk_server_submit(frk_server* s, uint32_t n_queries, const float* queries, uint32_t dim,
+                      const char* const* filters, const uint32_t* ks, uint64_t* ticket);
+/* Blocks until the request is done. times[3] = submitted, started, finished (optional);
+ * batch_queries = queries in the device batch that served it (optional). */
+int frk_server_wait(frk_server* s, uint64_t ticket, uint32_t k_stride, int64_t* out_ids, float* out_scores,
+                    uint32_t* out_counts, uint32_t* out_passing, double* times, uint32_t* batch_queries);
+int frk_server_stats_get(const frk_server* s, frk_server_stats* out);
+/* The device working set the scheduler charges against the HBM budget for a batch of n queries. */
+uint64_t frk_server_batch_bytes(const frk_server* s, uint32_t n_queries);
+double frk_server_now(const frk_server* s);
 
 #ifdef __cplusplus
 }
