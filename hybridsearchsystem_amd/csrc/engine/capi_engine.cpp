@@ -289,6 +289,7 @@ int frk_server_create(const frk_table* t, const char* vector_field, const char* 
         if (cfg->max_batch_queries) c.maxBatchQueries = cfg->max_batch_queries;
         if (cfg->workers) c.workers = cfg->workers;
         c.hbmBudgetBytes = cfg->hbm_budget_bytes;
+        if (cfg->pad_queries) c.padQueries = cfg->pad_queries;
     }
     VectorFieldData* vf = nullptr;
     if (!t->table->GetVectorFieldData(c.vectorField, vf) || !vf)

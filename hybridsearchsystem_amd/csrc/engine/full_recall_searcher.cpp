@@ -70,11 +70,11 @@ class SearchWorkspace {
         return std::find(m_seen.begin(), m_seen.end(), key) != m_seen.end();
     }
     void MarkSeen(const std::vector<uint64_t>& key) {
-        if (m_seen.size() >= 32) m_seen.erase(m_seen.begin());
+        if (m_seen.size() >= 128) m_seen.erase(m_seen.begin());
         m_seen.push_back(key);
     }
     void AddGraph(const std::vector<uint64_t>& key, hipGraphExec_t exec) {
-        if (m_graphs.size() >= 16) {
+        if (m_graphs.size() >= 64) {
             (void)hipGraphExecDestroy(m_graphs.front().exec);
             m_graphs.erase(m_graphs.begin());
         }
@@ -361,7 +361,10 @@ bool FullRecallSearcher::RunDevice(const VectorFieldData& vf, std::vector<AggrDe
     if (cap == 0) {
         cap = useProbe ? static_cast<uint32_t>(std::max<uint64_t>(65536, 16ull * kSel * probeStep))
                        : static_cast<uint32_t>(std::min<uint64_t>(nDocs, std::max<uint64_t>(65536, 64ull * kSel)));
-        if (floorChecked) cap = static_cast<uint32_t>(std::max<uint64_t>(cap, 65536));
+        // a caller or learned floor sits a calibrated margin below the k-th score, so the survivors
+        // above it can number many times k; room for 64 k (at least 256 Ki keys) keeps overflow
+        // reruns rare
+        if (floorChecked) cap = static_cast<uint32_t>(std::max<uint64_t>(cap, std::max<uint64_t>(262144, 64ull * kSel)));
         cap = std::min<uint32_t>(cap, std::max<uint32_t>(nDocs, 1));
     }
     const uint64_t probeBlocks = useProbe ? (nBlocks + probeStep - 1) / probeStep : 0;

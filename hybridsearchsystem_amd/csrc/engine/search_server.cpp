@@ -17,6 +17,23 @@ SearchServer::SearchServer(std::shared_ptr<DataTable> table, int32_t deviceId, c
         FRK_LOG_ERROR("server: no vector field " << m_cfg.vectorField);
     }
     m_vf = vf;
+    // an inert token: absent from the posting field, so a term on it matches no doc
+    PostingFieldData* pf = nullptr;
+    if (m_table && m_table->GetPostingFieldData(m_cfg.postingField, pf) && pf) {
+        for (uint64_t tok = 0xFFFFFFFFFFFFFFFFull;; tok--) {
+            bool exist = true;
+            (void)pf->GetPostingDeviceAddrs(tok, exist);
+            if (!exist) {
+                m_inertToken = tok;
+                break;
+            }
+        }
+    } else {
+        m_cfg.padQueries = 1;
+    }
+    m_cfg.padQueries = std::max<uint32_t>(1, m_cfg.padQueries);
+    m_inertTrees.resize(m_cfg.padQueries);
+    for (auto& t : m_inertTrees) t = std::make_unique<TermNode>(m_cfg.postingField, m_inertToken);
     m_cfg.workers = std::max<uint32_t>(1, m_cfg.workers);
     m_cfg.maxBatchQueries = std::max<uint32_t>(1, m_cfg.maxBatchQueries);
     m_slot.resize(m_cfg.workers);
@@ -44,13 +61,14 @@ double SearchServer::Now() const {
 
 uint64_t SearchServer::BatchBytes(uint32_t queries) const {
     // the searcher's per-batch buffers: query-major filter words, block-major masks, candidate lists
-    // (at least 65,536 keys per query), the k*m / rescored / output keys (up to 4,096 per query)
+    // (up to 262,144 keys per query with a learned floor), the k*m / rescored / output keys (up to
+    // 4,096 per query)
     const uint64_t n = m_table ? m_table->GetDocNum() : 0;
     const uint64_t q = queries, q32 = (q + 31) / 32 * 32;
     const uint64_t segWords = m_table ? static_cast<uint64_t>(m_table->GetSegmentNum()) *
                                             (m_table->GetDocNumPerSegment() / 64)
                                       : 0;
-    return q * segWords * 8 + q32 * ((n + 31) / 32) * 4 + q * 65536 * 8 + 3 * q * 4096 * 8;
+    return q * segWords * 8 + q32 * ((n + 31) / 32) * 4 + q * 262144 * 8 + 3 * q * 4096 * 8;
 }
 
 uint64_t SearchServer::Submit(uint32_t n, const float* queries, uint32_t dim, const std::vector<std::string>& filters,
@@ -181,11 +199,19 @@ void SearchServer::RunBatch(Batch& b) {
     const double started = Now();
     FullRecallSearcher s(m_deviceId, m_table);
     s.SetOptions(m_cfg.options);
-    for (auto& r : b.reqs)
+    uint32_t dim = 0;
+    for (auto& r : b.reqs) {
+        dim = r->dim;
         for (uint32_t i = 0; i < r->n; i++)
             s.AddQuery(r->trees[i].get(), &r->queries[static_cast<size_t>(i) * r->dim], r->dim, r->ks[i]);
+    }
+    // padding: zero vectors with k = 1 over an inert term (no doc passes, no survivors, no floor)
+    const uint32_t pad = (m_cfg.padQueries - b.queries % m_cfg.padQueries) % m_cfg.padQueries;
+    const std::vector<float> zeros(dim, 0.0f);
+    for (uint32_t i = 0; i < pad; i++) s.AddQuery(m_inertTrees[i].get(), zeros.data(), dim, 1);
     std::vector<std::shared_ptr<FullRecallResult>> res;
-    const bool ok = m_vf != nullptr && s.BatchSearch(res, m_cfg.vectorField, false) && res.size() == b.queries;
+    const bool ok =
+        m_vf != nullptr && s.BatchSearch(res, m_cfg.vectorField, false) && res.size() == b.queries + pad;
     const double finished = Now();
     const BatchStats st = s.LastStats();
     const std::vector<uint32_t>& passing = s.LastPassing();

@@ -33,6 +33,8 @@ struct ServerConfig {
     uint32_t maxBatchQueries = 256;   // queries per device batch
     uint32_t workers = 2;             // batches in flight on the device
     uint64_t hbmBudgetBytes = 0;      // device working memory for all batches in flight; 0 = unlimited
+    uint32_t padQueries = 64;         // batches are padded with inert queries to a multiple of this,
+                                      // so replayable graph shapes repeat; 1 = no padding
 };
 
 struct RequestResult {
@@ -91,6 +93,8 @@ class SearchServer {
 
     std::shared_ptr<DataTable> m_table;
     int32_t m_deviceId;
+    uint64_t m_inertToken = 0;                 // a token with no postings: padding queries match nothing
+    std::vector<std::unique_ptr<QueryNode>> m_inertTrees;
     ServerConfig m_cfg;
     const VectorFieldData* m_vf = nullptr;
     std::chrono::steady_clock::time_point m_t0;

@@ -34,6 +34,7 @@ typedef unsigned int u32x4 __attribute__((ext_vector_type(4)));
 
 constexpr int kThreads = 256;
 constexpr int kWaves = kThreads / 64;
+constexpr uint32_t kXcds = 8;  // accelerator complex dies of an MI355X, each with its own L2
 #ifndef FRK_SCORE_MIN_BLOCKS
 #define FRK_SCORE_MIN_BLOCKS 1  // workgroups per CU the register budget is sized for
 #endif
@@ -98,6 +99,7 @@ struct ScoreArgs {
     uint64_t* cand;
     uint32_t cap;
     uint32_t* counts;
+    uint32_t n_groups;  // query groups of 32 RB rows (set by the launch)
 };
 
 // Writes the query tile into LDS in the doc tiling, converted to the store's element type; for fp8 the
@@ -443,7 +445,22 @@ __global__ __launch_bounds__(kThreads, FRK_SCORE_MIN_BLOCKS) void ScoreKernel(Sc
     float* floor_lds = q_scale_lds + 32 * RB;                          // [32 * RB]
     uint32_t* masks_lds = reinterpret_cast<uint32_t*>(floor_lds + 32 * RB);  // [waves][32 * RB]
     uint64_t* queue_lds = reinterpret_cast<uint64_t*>(masks_lds + kWaves * 32 * RB);  // [waves][keys + runs]
-    const uint32_t q0 = blockIdx.y * (32 * RB);
+    // Workgroup -> (doc range x, query group): with several query groups every doc range is read once
+    // per group, so the groups of one range go to the same XCD (workgroups are dealt to the 8 XCDs
+    // round-robin by linear id) and run back to back there, sharing that XCD's L2 instead of each
+    // fetching the range from HBM. The grid is 1-D, 8 * groups * (ranges / 8) workgroups.
+    const uint32_t groups = A.n_groups;
+    const uint32_t n_ranges = gridDim.x / groups;
+    uint32_t range_x, group;
+    if (groups == 1) {
+        range_x = blockIdx.x;
+        group = 0;
+    } else {
+        const uint32_t xcd = blockIdx.x % kXcds, slot = blockIdx.x / kXcds;
+        group = slot % groups;
+        range_x = (slot / groups) * kXcds + xcd;
+    }
+    const uint32_t q0 = group * (32 * RB);
     StageQueries<DT, CH, RB>(A, a_lds, q_scale_lds, q0);
     for (int r = threadIdx.x; r < 32 * RB; r += kThreads) {
         const uint32_t q = q0 + r;
@@ -459,8 +476,8 @@ __global__ __launch_bounds__(kThreads, FRK_SCORE_MIN_BLOCKS) void ScoreKernel(Sc
     Qu.keys = queue_lds + wave * (kQueueKeys + kQueueRuns);
     Qu.runs = Qu.keys + kQueueKeys;
     Qu.nk = Qu.nr = 0;
-    const uint64_t per_wg = (A.block_count + gridDim.x - 1) / gridDim.x;
-    const uint64_t i_lo = static_cast<uint64_t>(blockIdx.x) * per_wg;
+    const uint64_t per_wg = (A.block_count + n_ranges - 1) / n_ranges;
+    const uint64_t i_lo = static_cast<uint64_t>(range_x) * per_wg;
     const uint64_t i_hi = i_lo + per_wg < A.block_count ? i_lo + per_wg : A.block_count;
     if (i_lo + wave >= i_hi) return;
     const uint64_t n_mine = (i_hi - i_lo - wave + kWaves - 1) / kWaves;  // blocks of this wave
@@ -554,22 +571,31 @@ int Launch(const ScoreArgs& A, hipStream_t stream, uint32_t grid_x) {
     const uint32_t resident = static_cast<uint32_t>(cus > 0 ? cus : 1) * static_cast<uint32_t>(resident_per_cu);
     const uint32_t per_group = resident / groups > 0 ? resident / groups : 1;
     if (grid_x > per_group) grid_x = per_group;
-    hipLaunchKernelGGL((ScoreKernel<DT, CH, RB, kBlockMax>), dim3(grid_x, groups), dim3(kThreads), lds, stream, A);
+    // whole rounds of the 8 XCDs, rounded down so ranges x groups stays within one resident wave
+    if (groups > 1) grid_x = grid_x >= kXcds ? grid_x / kXcds * kXcds : kXcds;
+    ScoreArgs a = A;
+    a.n_groups = groups;
+    hipLaunchKernelGGL((ScoreKernel<DT, CH, RB, kBlockMax>), dim3(grid_x * groups), dim3(kThreads), lds, stream, a);
     FRK_HIP_CHECK(hipGetLastError());
     return FRK_OK;
 }
 
-// Instantiated shapes: CH = 16-byte chunks per doc. RB = 32-query row blocks per workgroup, chosen so
-// the query tile fits the 160 KiB LDS.
+// Instantiated shapes: CH = 16-byte chunks per doc (FP32 dims 32..1024, BF16 64..2048, FP8 128..4096
+// at the listed counts). RB = 32-query row blocks per workgroup, at most what keeps the query tile
+// (RB * CH KiB) within 128 KiB of the 160 KiB LDS.
 template <int DT, bool kBlockMax>
 int Dispatch(const ScoreArgs& A, uint32_t ch, hipStream_t stream, uint32_t grid_x) {
     const uint32_t q = A.n_queries;
     const int rb = q > 64 ? 4 : (q > 32 ? 2 : 1);
-#define FRK_CASE(CHV)                                                              \
-    if (ch == CHV) {                                                               \
-        if (rb == 4 && CHV * 4 <= 128) return Launch<DT, CHV, 4, kBlockMax>(A, stream, grid_x); \
-        if (rb >= 2 && CHV * 2 <= 128) return Launch<DT, CHV, 2, kBlockMax>(A, stream, grid_x); \
-        return Launch<DT, CHV, 1, kBlockMax>(A, stream, grid_x);                              \
+#define FRK_CASE(CHV)                                                                           \
+    if (ch == CHV) {                                                                            \
+        if constexpr (CHV * 4 <= 128) {                                                         \
+            if (rb == 4) return Launch<DT, CHV, 4, kBlockMax>(A, stream, grid_x);               \
+        }                                                                                       \
+        if constexpr (CHV * 2 <= 128) {                                                         \
+            if (rb >= 2) return Launch<DT, CHV, 2, kBlockMax>(A, stream, grid_x);               \
+        }                                                                                       \
+        return Launch<DT, CHV, 1, kBlockMax>(A, stream, grid_x);                                \
     }
     FRK_CASE(4)
     FRK_CASE(8)
@@ -577,6 +603,8 @@ int Dispatch(const ScoreArgs& A, uint32_t ch, hipStream_t stream, uint32_t grid_
     FRK_CASE(32)
     FRK_CASE(48)
     FRK_CASE(64)
+    FRK_CASE(96)
+    FRK_CASE(128)
 #undef FRK_CASE
     SetError("frk_score_select: %u chunks per doc is not an instantiated shape", ch);
     return FRK_ERR_UNSUPPORTED;

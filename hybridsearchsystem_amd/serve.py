@@ -29,7 +29,7 @@ HBM_PEAK = 8.0e12
 
 class ServerConfig(ctypes.Structure):
     _fields_ = [("max_batch_queries", ctypes.c_uint32), ("workers", ctypes.c_uint32),
-                ("hbm_budget_bytes", ctypes.c_uint64)]
+                ("hbm_budget_bytes", ctypes.c_uint64), ("pad_queries", ctypes.c_uint32)]
 
 
 class ServerStats(ctypes.Structure):
@@ -63,14 +63,14 @@ def _bind(L):
 
 class Server:
     def __init__(self, table, dtype=0, rescore_factor=4, probe=True, floor_model=None, graph=True,
-                 max_batch_queries=256, workers=2, hbm_budget_bytes=0):
+                 max_batch_queries=256, workers=2, hbm_budget_bytes=0, pad_queries=64):
         L = _bind(lib())
         self.table = table
         self._model = floor_model.c_model() if floor_model is not None else None  # must outlive the server
         opts = SearchOptions(dtype, rescore_factor, int(bool(probe)), 0, None, 0, None, int(bool(graph)))
         if self._model is not None:
             opts.floor_model = ctypes.cast(ctypes.byref(self._model), ctypes.c_void_p)
-        cfg = ServerConfig(max_batch_queries, workers, hbm_budget_bytes)
+        cfg = ServerConfig(max_batch_queries, workers, hbm_budget_bytes, pad_queries)
         h = ctypes.c_void_p()
         check(L.frk_server_create(table.h, table.vector_field.encode(), table.posting_field.encode(),
                                   ctypes.byref(opts), ctypes.byref(cfg), ctypes.byref(h)), "frk_server_create")
@@ -211,7 +211,8 @@ def main():
     tq = synth.mixture_queries(S["train_queries"], args.train, args.dim, cents, corpus_seed=seed)
     tf, _ = synth.random_filters(S["train_filters"], dens, args.train, depth=4, sel=(1e-5, 0.6))
     tk = synth.draw_ks(S["train_kdraw"], args.train, [10, 100, 1000])
-    model, info = learned.train_for_table(table, args.dim, tq, tf, tk, log=log)
+    # a rerun costs a whole extra pass over the corpus, so the serving floor targets few of them
+    model, info = learned.train_for_table(table, args.dim, tq, tf, tk, val_fraction=0.1, fallback=0.001, log=log)
     log("learned floor trained: %s" % info)
     pool = 8192
     qs = synth.mixture_queries(S["eval_queries"], pool, args.dim, cents, corpus_seed=seed)

@@ -331,6 +331,8 @@ typedef struct {
     uint32_t max_batch_queries; /* 0 = 256 */
     uint32_t workers;           /* batches in flight; 0 = 2 */
     uint64_t hbm_budget_bytes;  /* 0 = unlimited */
+    uint32_t pad_queries;       /* pad batches with inert queries to a multiple of this (graph reuse);
+                                   0 = 64, 1 = no padding */
 } frk_server_config;
 
 typedef struct {

@@ -220,7 +220,8 @@ def _check_against_truth(frk, docs, qs, bits, ks, keys, oc, rel_tol=1e-5):
                 assert abs(float(d64[d] @ qs[q].astype(np.float64)) - kth) <= 2 * allowed, (q, d)
 
 
-@pytest.mark.parametrize("n,dim,nq", [(5000, 128, 64), (4096 + 17, 256, 100), (1000, 64, 7), (20000, 256, 128)])
+@pytest.mark.parametrize("n,dim,nq", [(5000, 128, 64), (4096 + 17, 256, 100), (1000, 64, 7), (20000, 256, 128),
+                                     (30000, 64, 300)])  # 300 queries: three query groups per doc range
 def test_fp32_topk_matches_cpu_truth(frk, n, dim, nq):
     print("fp32 topk n=%d dim=%d nq=%d" % (n, dim, nq), flush=True)
     rng = np.random.default_rng(n + dim + nq)
