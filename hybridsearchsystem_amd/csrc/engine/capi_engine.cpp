@@ -345,6 +345,7 @@ int frk_server_stats_get(const frk_server* s, frk_server_stats* out) {
     out->batches = st.batches;
     out->graph_replays = st.graphReplays;
     out->reruns = st.reruns;
+    out->overflows = st.overflows;
     out->device_ms = st.deviceMs;
     out->doc_bytes_streamed = st.docBytesStreamed;
     return FRK_OK;

@@ -341,7 +341,13 @@ bool FullRecallSearcher::RunDevice(const VectorFieldData& vf, std::vector<AggrDe
             // too few docs expected to pass for a floor to pay off (f[2] = log10 expected passing)
             const bool allow = std::pow(10.0, static_cast<double>(f[2])) >=
                                static_cast<double>(model->min_passing_ratio) * m_topKs[q] * m;
-            hoff[q] = allow ? -model->margin : -std::numeric_limits<float>::infinity();
+            // margin of this k's decade (f[1] = log10 k) when the model has per-decade margins
+            float margin = model->margin;
+            if (model->margin_by_k) {
+                const int b = std::min(3, std::max(0, static_cast<int>(std::floor(f[1] + 1e-4f))));
+                margin = model->margin_by_k[b];
+            }
+            hoff[q] = allow ? -margin : -std::numeric_limits<float>::infinity();
         }
     }
     const bool floorChecked = callerFloor || learned;  // a floor that may cut below k: verify, rerun
@@ -362,9 +368,9 @@ bool FullRecallSearcher::RunDevice(const VectorFieldData& vf, std::vector<AggrDe
         cap = useProbe ? static_cast<uint32_t>(std::max<uint64_t>(65536, 16ull * kSel * probeStep))
                        : static_cast<uint32_t>(std::min<uint64_t>(nDocs, std::max<uint64_t>(65536, 64ull * kSel)));
         // a caller or learned floor sits a calibrated margin below the k-th score, so the survivors
-        // above it can number many times k; room for 64 k (at least 256 Ki keys) keeps overflow
-        // reruns rare
-        if (floorChecked) cap = static_cast<uint32_t>(std::max<uint64_t>(cap, std::max<uint64_t>(262144, 64ull * kSel)));
+        // above it can number many times k (where scores crowd near the k-th best); room for 256 k
+        // keys (at least 256 Ki) keeps overflow reruns, each a whole extra pass, rare
+        if (floorChecked) cap = static_cast<uint32_t>(std::max<uint64_t>(cap, std::max<uint64_t>(262144, 256ull * kSel)));
         cap = std::min<uint32_t>(cap, std::max<uint32_t>(nDocs, 1));
     }
     const uint64_t probeBlocks = useProbe ? (nBlocks + probeStep - 1) / probeStep : 0;

@@ -61,14 +61,15 @@ double SearchServer::Now() const {
 
 uint64_t SearchServer::BatchBytes(uint32_t queries) const {
     // the searcher's per-batch buffers: query-major filter words, block-major masks, candidate lists
-    // (up to 262,144 keys per query with a learned floor), the k*m / rescored / output keys (up to
+    // (up to 256 k keys per query with a learned floor), the k*m / rescored / output keys (up to
     // 4,096 per query)
     const uint64_t n = m_table ? m_table->GetDocNum() : 0;
     const uint64_t q = queries, q32 = (q + 31) / 32 * 32;
     const uint64_t segWords = m_table ? static_cast<uint64_t>(m_table->GetSegmentNum()) *
                                             (m_table->GetDocNumPerSegment() / 64)
                                       : 0;
-    return q * segWords * 8 + q32 * ((n + 31) / 32) * 4 + q * 262144 * 8 + 3 * q * 4096 * 8;
+    const uint64_t cand = std::min<uint64_t>(std::max<uint64_t>(n, 1), 256ull * 4096);
+    return q * segWords * 8 + q32 * ((n + 31) / 32) * 4 + q * cand * 8 + 3 * q * 4096 * 8;
 }
 
 uint64_t SearchServer::Submit(uint32_t n, const float* queries, uint32_t dim, const std::vector<std::string>& filters,
@@ -244,6 +245,7 @@ void SearchServer::RunBatch(Batch& b) {
     m_stats.batches++;
     m_stats.graphReplays += st.graphReplayed ? 1 : 0;
     m_stats.reruns += st.reruns;
+    m_stats.overflows += st.overflowed;
     m_stats.deviceMs += st.ms_total;
     m_stats.docBytesStreamed += docBytes;
 }

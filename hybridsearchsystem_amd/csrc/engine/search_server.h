@@ -48,7 +48,7 @@ struct RequestResult {
 };
 
 struct ServerStats {
-    uint64_t requests = 0, queries = 0, batches = 0, graphReplays = 0, reruns = 0;
+    uint64_t requests = 0, queries = 0, batches = 0, graphReplays = 0, reruns = 0, overflows = 0;
     double deviceMs = 0;       // sum of per-batch device time (events around each batch)
     uint64_t docBytesStreamed = 0;  // doc-store bytes the main score passes read
 };

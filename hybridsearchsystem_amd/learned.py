@@ -112,7 +112,12 @@ def expected_passing(feats):
 class FloorModelC(ctypes.Structure):
     """frk_floor_model of include/frk_capi.h."""
     _fields_ = [("mlp", MlpParams), ("feat_mean", ctypes.c_void_p), ("feat_std", ctypes.c_void_p),
-                ("margin", ctypes.c_float), ("min_passing_ratio", ctypes.c_float)]
+                ("margin", ctypes.c_float), ("min_passing_ratio", ctypes.c_float), ("margin_by_k", ctypes.c_void_p)]
+
+
+def k_decade(feats):
+    """Decade bucket of k (0: k < 10, 1: < 100, 2: < 1000, 3: >= 1000) from the log10 k feature."""
+    return np.clip(np.floor(np.asarray(feats, np.float32)[:, 1] + 1e-4), 0, 3).astype(np.int64)
 
 
 # ------------------------------------------------------------------------------------------- model
@@ -140,6 +145,7 @@ class FloorModel:
         self.feat_mean = np.zeros(N_FEAT, np.float32)
         self.feat_std = np.ones(N_FEAT, np.float32)
         self.margin = 0.0
+        self.margin_by_k = None  # optional float32 [4]: per decade of k
         self.min_passing_ratio = 2.0
         self._ws = {}
 
@@ -162,9 +168,10 @@ class FloorModel:
         holds references to the host arrays it points at."""
         mean = np.ascontiguousarray(self.feat_mean, np.float32)
         std = np.ascontiguousarray(self.feat_std, np.float32)
+        mk = None if self.margin_by_k is None else np.ascontiguousarray(self.margin_by_k, np.float32)
         m = FloorModelC(self.params_struct(), mean.ctypes.data, std.ctypes.data, float(self.margin),
-                        float(self.min_passing_ratio))
-        m._keep = (mean, std)
+                        float(self.min_passing_ratio), None if mk is None else mk.ctypes.data)
+        m._keep = (mean, std, mk)
         return m
 
     def inputs(self, queries, feats):
@@ -250,19 +257,32 @@ class FloorModel:
             out[lo:lo + 8192] = self.forward(x[lo:lo + 8192].contiguous()).cpu().numpy()
         return out
 
-    def calibrate(self, queries, feats, labels, fallback=0.01):
-        """Margin so that at most `fallback` of the held-out queries get a floor above their label."""
+    def calibrate(self, queries, feats, labels, fallback=0.01, per_k=True):
+        """Margin so that at most `fallback` of the held-out queries get a floor above their label;
+        with per_k, one margin per decade of k (the score density near the k-th best, and with it
+        the cost of a loose floor, differs by orders of magnitude between k = 10 and k = 4000),
+        falling back to the global margin for a decade with too few held-out queries."""
         ok = np.isfinite(labels)
-        pred = self.predict_raw(np.asarray(queries)[ok], np.asarray(feats)[ok])
+        feats = np.asarray(feats)[ok]
+        pred = self.predict_raw(np.asarray(queries)[ok], feats)
         over = pred - np.asarray(labels, np.float32)[ok]
         self.margin = float(max(np.quantile(over, 1.0 - fallback), 0.0))
+        if per_k:
+            dec = k_decade(feats)
+            mk = np.full(4, self.margin, np.float32)
+            for b in range(4):
+                sel = over[dec == b]
+                if len(sel) >= max(50, int(2.0 / fallback)):
+                    mk[b] = max(float(np.quantile(sel, 1.0 - fallback)), 0.0)
+            self.margin_by_k = mk
         return self.margin
 
     def floors(self, queries, feats, ks):
         """Per-query floors on the host, as the searcher computes them on the device: prediction minus
         the margin; -inf where fewer than min_passing_ratio * k docs are expected to pass (too little
         to prune, too easy to overshoot)."""
-        fl = self.predict_raw(queries, feats) - self.margin
+        margin = self.margin if self.margin_by_k is None else np.asarray(self.margin_by_k)[k_decade(feats)]
+        fl = self.predict_raw(queries, feats) - margin
         fl[expected_passing(feats) < self.min_passing_ratio * np.asarray(ks, np.float64)] = -np.inf
         return fl.astype(np.float32)
 
@@ -270,7 +290,8 @@ class FloorModel:
     def state(self):
         st = {k: v.cpu().numpy() for k, v in self.p.items()}
         st.update(feat_mean=self.feat_mean, feat_std=self.feat_std, margin=np.float32(self.margin),
-                  tau=np.float32(self.tau))
+                  tau=np.float32(self.tau),
+                  margin_by_k=np.full(4, np.nan, np.float32) if self.margin_by_k is None else self.margin_by_k)
         return st
 
     def load_state(self, st):
@@ -280,6 +301,8 @@ class FloorModel:
         self.feat_mean = np.asarray(st["feat_mean"], np.float32)
         self.feat_std = np.asarray(st["feat_std"], np.float32)
         self.margin = float(st["margin"])
+        mk = np.asarray(st.get("margin_by_k", np.full(4, np.nan)), np.float32)
+        self.margin_by_k = None if np.isnan(mk).any() else mk
 
 
 def kth_labels(table, queries, filters, ks, batch=128, **search_kw):
@@ -323,5 +346,6 @@ def train_for_table(table, dim, queries, filters, ks, rescore_factors=(1, 4), se
     t2 = time.time()
     info = {"train_queries": int(np.isfinite(labels[tr]).sum()), "val_queries": int(np.isfinite(labels[va]).sum()),
             "label_s": round(t1 - t0, 2), "train_s": round(t2 - t1, 2), "loss": [round(x, 5) for x in hist],
-            "margin": round(model.margin, 5), "target_fallback": fallback}
+            "margin": round(model.margin, 5), "target_fallback": fallback,
+            "margin_by_k_decade": None if model.margin_by_k is None else [round(float(x), 5) for x in model.margin_by_k]}
     return model, info

@@ -34,7 +34,8 @@ class ServerConfig(ctypes.Structure):
 
 class ServerStats(ctypes.Structure):
     _fields_ = [("requests", ctypes.c_uint64), ("queries", ctypes.c_uint64), ("batches", ctypes.c_uint64),
-                ("graph_replays", ctypes.c_uint64), ("reruns", ctypes.c_uint64), ("device_ms", ctypes.c_double),
+                ("graph_replays", ctypes.c_uint64), ("reruns", ctypes.c_uint64), ("overflows", ctypes.c_uint64),
+                ("device_ms", ctypes.c_double),
                 ("doc_bytes_streamed", ctypes.c_uint64)]
 
 
@@ -173,7 +174,7 @@ def poisson_load(server, queries, filters, rate_qps, n_requests, max_request=256
            "p50_ms": round(float(np.percentile(lat, 50)), 3), "p99_ms": round(float(np.percentile(lat, 99)), 3),
            "batches": s1["batches"] - s0["batches"], "graph_replays": s1["graph_replays"] - s0["graph_replays"],
            "mean_batch_queries": round(q / max(1, s1["batches"] - s0["batches"]), 1),
-           "reruns": s1["reruns"] - s0["reruns"],
+           "reruns": s1["reruns"] - s0["reruns"], "overflows": s1["overflows"] - s0["overflows"],
            "doc_stream_gbs": round(doc_bytes / wall / 1e9, 1),
            "hbm_roofline_frac": round(doc_bytes / wall / HBM_PEAK, 4)}
     return out, reqs, done
@@ -190,6 +191,7 @@ def main():
     ap.add_argument("--train", type=int, default=50_000, help="learned-floor training queries")
     ap.add_argument("--workers", type=int, default=2)
     ap.add_argument("--plain", action="store_true", help="also run the plain FP32 path at each rate")
+    ap.add_argument("--diag", action="store_true", help="compare host- and device-side learned floors first")
     args = ap.parse_args()
     import torch
     from . import learned, search, synth
@@ -212,11 +214,38 @@ def main():
     tf, _ = synth.random_filters(S["train_filters"], dens, args.train, depth=4, sel=(1e-5, 0.6))
     tk = synth.draw_ks(S["train_kdraw"], args.train, [10, 100, 1000])
     # a rerun costs a whole extra pass over the corpus, so the serving floor targets few of them
-    model, info = learned.train_for_table(table, args.dim, tq, tf, tk, val_fraction=0.1, fallback=0.001, log=log)
+    model, info = learned.train_for_table(table, args.dim, tq, tf, tk, val_fraction=0.2, fallback=0.001, log=log)
     log("learned floor trained: %s" % info)
     pool = 8192
     qs = synth.mixture_queries(S["eval_queries"], pool, args.dim, cents, corpus_seed=seed)
     fs, _ = synth.random_filters(S["eval_filters"], dens, pool, depth=4, sel=(1e-5, 0.6))
+    # how often the floor would sit above the true k'-th score on the serving pool, per k' (k * 4)
+    diag = {}
+    for k in (10, 100, 1000):
+        kp = np.full(pool, 4 * k, np.uint32)
+        lab, pas = learned.kth_labels(table, qs, fs, kp)
+        fl = model.floors(qs, learned.features(table, qs, fs, kp), kp)
+        live = np.isfinite(fl)
+        over = live & (fl > lab)
+        diag[4 * k] = {"floored": int(live.sum()), "above_kth": int(over.sum()),
+                       "above_kth_with_fewer_passing": int((over & (pas < kp)).sum())}
+    log("floor check on the serving pool: %s" % diag)
+    if args.diag:
+        ks = synth.draw_ks(5, 1024, [10, 100, 1000]).astype(np.uint32)
+        fl = model.floors(qs[:1024], learned.features(table, qs[:1024], fs[:1024], ks * 4), ks * 4)
+        for name, kw in (("host floors", dict(floors=None)), ("device floors", dict(floor_model=model)),
+                         ("device floors + graph", dict(floor_model=model, graph=True))):
+            rr = ov = 0
+            for lo in range(0, 1024, 128):
+                sl = slice(lo, lo + 128)
+                extra = dict(kw)
+                if name == "host floors":
+                    extra["floors"] = fl[sl]
+                r = table.search(qs[sl], fs[sl], ks[sl], dtype=search.FRK_DTYPE_BF16, rescore_factor=4, probe=False,
+                                 **extra)
+                rr += r[4]["reruns"]
+                ov += r[4]["overflowed"]
+            log("diag %s: reruns %d overflowed %d of 1024" % (name, rr, ov))
     runs = [("bf16 + learned floor + fp32 rescore, graph replay", args.rates,
              dict(dtype=search.FRK_DTYPE_BF16, rescore_factor=4, probe=False, floor_model=model, graph=True))]
     if args.plain:

@@ -226,15 +226,17 @@ int frk_mlp_forward(const frk_mlp_params* p, int32_t rows, const float* d_x, int
 int frk_mlp_floors(int32_t rows, const float* d_y, const float* d_offset, float* d_floors, frk_stream_t stream);
 
 /* A trained floor model as the searcher uses it: input row = [query | (features - mean) / std],
- * floor = prediction - margin, no floor where fewer than min_passing_ratio * k docs are expected to
- * pass (k = the keys the main pass selects, k * rescore_factor on low precision). */
+ * floor = prediction - margin (per decade of k when margin_by_k is given), no floor where fewer than
+ * min_passing_ratio * k docs are expected to pass (k = the keys the main pass selects,
+ * k * rescore_factor on low precision). */
 #define FRK_FLOOR_FEATURES 32
 typedef struct {
     frk_mlp_params mlp;       /* device weights; mlp.in_dim = query dim + FRK_FLOOR_FEATURES */
     const float* feat_mean;   /* host [FRK_FLOOR_FEATURES] */
     const float* feat_std;    /* host [FRK_FLOOR_FEATURES] */
-    float margin;
+    float margin;             /* for every k unless margin_by_k is set */
     float min_passing_ratio;
+    const float* margin_by_k; /* optional host [4]: margins for k < 10, < 100, < 1000, >= 1000 */
 } frk_floor_model;
 
 /* ================================================================================================
@@ -336,7 +338,7 @@ typedef struct {
 } frk_server_config;
 
 typedef struct {
-    uint64_t requests, queries, batches, graph_replays, reruns;
+    uint64_t requests, queries, batches, graph_replays, reruns, overflows; /* reruns include overflows */
     double device_ms;            /* sum over batches of the device time of each */
     uint64_t doc_bytes_streamed; /* doc-store bytes read by the main score passes */
 } frk_server_stats;
