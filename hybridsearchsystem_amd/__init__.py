@@ -46,6 +46,8 @@ def lib():
     L.frk_filter_eval.argtypes = [vp, u32, vp, vp, u32, u32, u64, vp, u64, vp, vp, u32, vp]
     L.frk_chunk_dims.restype = u32
     L.frk_chunk_dims.argtypes = [u32]
+    L.frk_store_dim.restype = u32
+    L.frk_store_dim.argtypes = [u32, u32]
     L.frk_score_select.restype = i32
     L.frk_score_select.argtypes = [ctypes.POINTER(DocStore), vp, u32, vp, u32, vp, u64, u64, u64, vp, u32, vp, vp]
     L.frk_select_topk.restype = i32

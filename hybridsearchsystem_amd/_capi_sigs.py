@@ -16,6 +16,15 @@ class SearchStats(ctypes.Structure):
                 ("cand_cap", u32), ("graph_replayed", u32)]
 
 
+class IndexVectorField(ctypes.Structure):
+    _fields_ = [("name", cp), ("dim", u32), ("file_type", u32), ("rows", vp)]
+
+
+class IndexPostingField(ctypes.Structure):
+    _fields_ = [("name", cp), ("is_matrix", ctypes.c_int32), ("density_threshold", f32), ("packed", ctypes.c_int32),
+                ("n_tokens", u64), ("tags", vp), ("terms", vp), ("offsets", vp), ("doc_ids", vp)]
+
+
 def bind(L):
     L.frk_tiles_bytes.restype = u64
     L.frk_tiles_bytes.argtypes = [u64, u32, u32]
@@ -53,6 +62,11 @@ def bind(L):
     L.frk_builder_set_gdocids.argtypes = [vp, vp, u64]
     L.frk_builder_finish.restype = i32
     L.frk_builder_finish.argtypes = [vp, ctypes.POINTER(vp)]
+    L.frk_write_index.restype = i32
+    L.frk_write_index.argtypes = [cp, u32, u32, u32, u32, u32, ctypes.POINTER(IndexVectorField), u32,
+                                  ctypes.POINTER(IndexPostingField), vp]
+    L.frk_f32_to_f16.restype = ctypes.c_uint16
+    L.frk_f32_to_f16.argtypes = [f32]
     L.frk_hash64.restype = u64
     L.frk_hash64.argtypes = [cp, u32]
     L.frk_serialize_posting.restype = ctypes.c_int64

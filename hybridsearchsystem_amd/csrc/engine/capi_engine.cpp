@@ -10,6 +10,7 @@
 #include "../../../include/frk_capi.h"
 #include "data_table.h"
 #include "full_recall_searcher.h"
+#include "index_writer.h"
 #include "search_server.h"
 #include "log.h"
 #include "query_node.h"
@@ -33,6 +34,7 @@ int Fail(int code, const char* what) {
     frk::SetError("%s", what);
     return code;
 }
+int Fail(int code, const std::string& what) { return Fail(code, what.c_str()); }
 
 void FilterToPairs(const FilterAst* n, std::vector<int64_t>& out) {
     switch (n->op) {
@@ -135,6 +137,31 @@ int frk_builder_finish(frk_builder* b, frk_table** out) {
     *out = new frk_table{t};
     return FRK_OK;
 }
+
+int frk_write_index(const char* dir, uint32_t version, uint32_t n_docs, uint32_t docs_per_segment,
+                    uint32_t split_doc_num_zn, uint32_t n_vector_fields, const frk_index_vector_field* vector_fields,
+                    uint32_t n_posting_fields, const frk_index_posting_field* posting_fields, const uint64_t* gdocids) {
+    if (!dir || (n_vector_fields && !vector_fields) || (n_posting_fields && !posting_fields))
+        return Fail(FRK_ERR_INVALID, "frk_write_index: null argument");
+    IndexWriter w(n_docs, docs_per_segment, version, split_doc_num_zn);
+    for (uint32_t i = 0; i < n_vector_fields; i++) {
+        const frk_index_vector_field& f = vector_fields[i];
+        if (!f.name || f.file_type > FRK_VECTOR_FILE_FP32_ROWS)
+            return Fail(FRK_ERR_INVALID, "frk_write_index: bad vector field");
+        w.AddVectorField(f.name, f.dim, f.rows, static_cast<VectorFileType>(f.file_type));
+    }
+    for (uint32_t i = 0; i < n_posting_fields; i++) {
+        const frk_index_posting_field& f = posting_fields[i];
+        if (!f.name) return Fail(FRK_ERR_INVALID, "frk_write_index: bad posting field");
+        w.AddPostingField(f.name, f.is_matrix != 0, f.density_threshold, f.packed != 0, f.n_tokens, f.tags, f.terms,
+                          f.offsets, f.doc_ids);
+    }
+    if (gdocids) w.SetGDocIds(gdocids, n_docs);
+    if (!w.Write(dir)) return Fail(FRK_ERR_INVALID, "frk_write_index: " + w.Error());
+    return FRK_OK;
+}
+
+uint16_t frk_f32_to_f16(float f) { return Float32ToFloat16(f); }
 
 uint64_t frk_hash64(const char* data, uint32_t len) { return Hash64(data, len); }
 

@@ -25,11 +25,7 @@ bool ReadFile(const std::string& path, std::vector<uint8_t>& out) {
     return static_cast<bool>(f);
 }
 
-uint32_t DimPad(uint32_t dim, uint32_t dtype) {
-    uint32_t cd = frk_chunk_dims(dtype);
-    if (dtype == FRK_DTYPE_FP8E4M3) cd *= 2;  // the FP8 MFMA step consumes two chunks
-    return (dim + cd - 1) / cd * cd;
-}
+uint32_t DimPad(uint32_t dim, uint32_t dtype) { return frk_store_dim(dim, dtype); }
 }  // namespace
 
 // ------------------------------------------------------------------------------------------ postings

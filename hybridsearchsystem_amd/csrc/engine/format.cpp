@@ -361,6 +361,24 @@ std::string PathJoin(const std::string& a, const std::string& b) {
     return a.back() == '/' ? a + b : a + "/" + b;
 }
 
+uint16_t Float32ToFloat16(float f) {
+    uint32_t u;
+    std::memcpy(&u, &f, 4);
+    const uint32_t sign = (u >> 16) & 0x8000u;
+    const uint32_t a = u & 0x7FFFFFFFu;
+    if (a >= 0x7F800000u) {  // NaN keeps its top payload bits; infinity (zero payload) becomes NaN 1
+        const uint32_t payload = (a >> 13) & 0x3FFu;
+        return static_cast<uint16_t>(sign | 0x7C00u | (payload ? payload : 1u));
+    }
+    if (a < 0x38800000u) return static_cast<uint16_t>(sign);  // below 2^-14 (FP32 subnormals too)
+    // rebias the exponent (127 -> 15), then round the 13 dropped bits to nearest even; a carry out of
+    // the mantissa moves into the exponent, and exponent 31 is infinity
+    uint32_t r = a - (112u << 23);
+    r += 0xFFFu + ((r >> 13) & 1u);
+    if (r >= (31u << 23)) return static_cast<uint16_t>(sign | 0x7C00u);
+    return static_cast<uint16_t>(sign | (r >> 13));
+}
+
 std::string SegmentFileName(uint32_t version, uint32_t segment, const std::string& field, const char* suffix) {
     std::string n = std::string(kPoissonEngine) + "_" + std::to_string(version) + "_" + std::to_string(segment);
     if (!field.empty()) n += "_" + field;

@@ -61,6 +61,11 @@ enum class VectorFileType : uint16_t { FP16_ZN = 0, FP32_ROWS = 1 };
 // Legacy FP16 codec of the reference's vector files: subnormals flush to zero both ways
 // (core/number/trans_number.cpp:84-105).
 float Float16ToFloat32(uint16_t h);
+// FP32 -> FP16 as the reference builder stores vectors (core/number/trans_number.cpp:38-82): round to
+// nearest even, no FP16 subnormals (magnitudes below 2^-14 become signed zero), finite overflow to
+// infinity, NaN stays NaN with its top payload bits, and an infinite input becomes the NaN 0x7C01
+// (sign kept), as the reference's exponent-255 branch does.
+uint16_t Float32ToFloat16(float f);
 
 std::string PathJoin(const std::string& a, const std::string& b);
 std::string SegmentFileName(uint32_t version, uint32_t segment, const std::string& field, const char* suffix);

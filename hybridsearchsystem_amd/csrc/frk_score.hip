@@ -152,6 +152,17 @@ extern "C" uint32_t frk_chunk_dims(uint32_t dtype) {
     return 0;
 }
 
+extern "C" uint32_t frk_store_dim(uint32_t dim, uint32_t dtype) {
+    const uint32_t cd = frk_chunk_dims(dtype);
+    if (cd == 0 || dim == 0) return 0;
+    const uint32_t step = dtype == FRK_DTYPE_FP8E4M3 ? 2 : 1;  // one FP8 MFMA step reads two chunks
+    uint32_t ch = (dim + cd - 1) / cd;
+    ch = (ch + step - 1) / step * step;
+    for (uint32_t s : frk::kScoreShapes)
+        if (s >= ch) return s * cd;
+    return ch * cd;  // beyond the largest scorer shape: the rescorer reads such a store, the scorer refuses it
+}
+
 extern "C" int frk_score_select(const frk_doc_store* store, const float* d_queries, uint32_t n_queries,
                                 const uint32_t* d_block_masks, uint32_t mask_q_stride, const float* d_floor,
                                 uint64_t block_begin, uint64_t block_step, uint64_t block_count, uint64_t* d_cand,

@@ -164,11 +164,13 @@ __device__ __forceinline__ void StaticFor(F&& f, std::integer_sequence<int, I...
     (f(std::integral_constant<int, I>{}), ...);
 }
 
+constexpr int Gcd(int a, int b) { return b == 0 ? a : Gcd(b, a % b); }
+
 template <int CH>
 struct GroupShape {
     static constexpr int G = CH < 4 ? CH : 4;                          // chunks per group
     static constexpr int kGroups = CH / G;                             // groups per doc block
-    static constexpr int kUnroll = kGroups >= kRing ? kGroups : kRing;  // steps per loop body
+    static constexpr int kUnroll = kGroups / Gcd(kGroups, kRing) * kRing;  // steps per loop body (lcm)
     static_assert(CH % G == 0, "chunks per doc must be a multiple of the group size");
     static_assert(kUnroll % kGroups == 0 && kUnroll % kRing == 0, "loop body must hold whole blocks and rings");
 };
@@ -550,6 +552,9 @@ __global__ __launch_bounds__(kThreads, FRK_SCORE_MIN_BLOCKS) void ScoreKernel(Sc
 
 using KernelFn = void (*)(ScoreArgs);
 
+// chunk counts per doc the scorer is compiled for, ascending (the cases of Dispatch below)
+constexpr uint32_t kScoreShapes[] = {4, 8, 12, 16, 24, 32, 48, 64, 96, 128};
+
 template <int DT, int CH, int RB, bool kBlockMax>
 int Launch(const ScoreArgs& A, hipStream_t stream, uint32_t grid_x) {
     const size_t lds = static_cast<size_t>(RB) * CH * 64 * sizeof(u32x4) + 2 * 32 * RB * sizeof(float) +
@@ -581,7 +586,7 @@ int Launch(const ScoreArgs& A, hipStream_t stream, uint32_t grid_x) {
 }
 
 // Instantiated shapes: CH = 16-byte chunks per doc (FP32 dims 32..1024, BF16 64..2048, FP8 128..4096
-// at the listed counts). RB = 32-query row blocks per workgroup, at most what keeps the query tile
+// at the listed counts; kScoreShapes lists them for frk_store_dim, which pads a store's dims up to one). RB = 32-query row blocks per workgroup, at most what keeps the query tile
 // (RB * CH KiB) within 128 KiB of the 160 KiB LDS.
 template <int DT, bool kBlockMax>
 int Dispatch(const ScoreArgs& A, uint32_t ch, hipStream_t stream, uint32_t grid_x) {
@@ -599,7 +604,9 @@ int Dispatch(const ScoreArgs& A, uint32_t ch, hipStream_t stream, uint32_t grid_
     }
     FRK_CASE(4)
     FRK_CASE(8)
+    FRK_CASE(12)
     FRK_CASE(16)
+    FRK_CASE(24)
     FRK_CASE(32)
     FRK_CASE(48)
     FRK_CASE(64)

@@ -17,10 +17,8 @@ def chunk_dims(dtype):
 
 
 def padded_dim(dim, dtype):
-    cd = chunk_dims(dtype)
-    if dtype == FRK_DTYPE_FP8E4M3:
-        cd *= 2  # the FP8 MFMA consumes two chunks per step
-    return (dim + cd - 1) // cd * cd
+    """The store dim the scorer runs over: dim zero-padded to the next compiled chunk count."""
+    return int(lib().frk_store_dim(dim, dtype))
 
 
 def tile_docs(vectors, dtype, dim_pad=None, threads=0):

@@ -1,17 +1,19 @@
 """Index tables and batch search through the C ABI handle layer (include/frk_capi.h, layer 2).
 
 ``Table.build`` stages an index in device memory from arrays (vectors host- or device-resident,
-per-tag posting lists as CSR); ``Table.load`` reads an index directory. ``Table.search`` runs one
-batch through ``NpuRetrieval::FullRecallSearcher``.
+per-tag posting lists as CSR); ``write_index`` writes the same arrays as an index directory on disk
+and ``Table.load`` reads one. ``Table.search`` runs one batch through
+``NpuRetrieval::FullRecallSearcher``.
 """
 import ctypes
 
 import numpy as np
 
 from . import check, lib
-from ._capi_sigs import SearchOptions, SearchStats
+from ._capi_sigs import IndexPostingField, IndexVectorField, SearchOptions, SearchStats
 
 FRK_DTYPE_F32, FRK_DTYPE_BF16, FRK_DTYPE_FP8E4M3 = 0, 1, 2
+VECTOR_FILE_FP16_ZN, VECTOR_FILE_FP32_ROWS = 0, 1
 
 
 def _mask(dtypes):
@@ -138,6 +140,42 @@ class Table:
                                cnt.ctypes.data, passing.ctypes.data, ctypes.byref(st)), "frk_search")
         stats = {f: getattr(st, f) for f, _ in SearchStats._fields_}
         return ids, sc, cnt, passing, stats
+
+
+def write_index(path, vectors, tag_offsets, tag_doc_ids, tags=None, terms=None, docs_per_segment=131072,
+                version=0, split_doc_num_zn=32, vector_file=VECTOR_FILE_FP16_ZN, vector_field="vec",
+                posting_field="tag", is_matrix=False, threshold=0.05, packed=True, gdocids=None):
+    """Writes an index directory (.meta, id/*.idm, <vector_field>/*.vector, <posting_field>/*.posting)
+    from host arrays: vectors [n, dim] float32; postings as CSR over tokens (tags[t] as decimal text,
+    or the strings in `terms`)."""
+    v = np.ascontiguousarray(vectors, dtype=np.float32)
+    n, dim = v.shape
+    off = np.ascontiguousarray(tag_offsets, dtype=np.uint64)
+    ids = np.ascontiguousarray(tag_doc_ids, dtype=np.uint32)
+    nt = len(off) - 1
+    t = np.ascontiguousarray(tags if tags is not None else np.arange(nt), dtype=np.uint64)
+    term_arr = None
+    if terms is not None:
+        term_arr = (ctypes.c_char_p * nt)(*[x.encode() for x in terms])
+    vf = IndexVectorField(vector_field.encode(), dim, vector_file, v.ctypes.data)
+    pf = IndexPostingField(posting_field.encode(), int(is_matrix), float(threshold), int(packed), nt, t.ctypes.data,
+                           ctypes.cast(term_arr, ctypes.c_void_p) if term_arr is not None else None, off.ctypes.data,
+                           ids.ctypes.data if len(ids) else None)
+    g = None
+    if gdocids is not None:
+        g = np.ascontiguousarray(gdocids, dtype=np.uint64)
+        if len(g) != n:
+            raise ValueError("gdocids must hold one id per doc")
+    check(lib().frk_write_index(str(path).encode(), version, n, docs_per_segment, split_doc_num_zn, 1,
+                                ctypes.byref(vf), 1, ctypes.byref(pf), g.ctypes.data if g is not None else None),
+          "frk_write_index")
+
+
+def f32_to_f16(x):
+    """The index writer's FP32 -> FP16 rounding, elementwise (uint16 bit patterns)."""
+    a = np.ascontiguousarray(x, dtype=np.float32).ravel()
+    f = lib().frk_f32_to_f16
+    return np.array([f(float(e)) for e in a], dtype=np.uint16).reshape(np.shape(x))
 
 
 def parse_filter(expr):

@@ -99,6 +99,10 @@ typedef struct {
 } frk_doc_store;
 
 uint32_t frk_chunk_dims(uint32_t dtype);
+/* The padded dim of a store the scorer runs over: dim rounded up, with zero dims, to the next chunk
+ * count the scorer is compiled for (4, 8, 12, 16, 24, 32, 48, 64, 96 or 128 chunks). Above 128
+ * chunks, the whole-chunk dim (such a store serves frk_rescore; frk_score_* refuse it). */
+uint32_t frk_store_dim(uint32_t dim, uint32_t dtype);
 
 /* d[i] = value for i < n (a kernel, so it may sit inside a captured graph). */
 int frk_fill_u32(uint32_t* d, uint64_t n, uint32_t value, frk_stream_t stream);
@@ -267,6 +271,41 @@ int frk_builder_add_tag_postings(frk_builder* b, const char* field, uint64_t n_t
                                  const uint64_t* offsets, const uint32_t* doc_ids);
 int frk_builder_set_gdocids(frk_builder* b, const uint64_t* ids, uint64_t n);
 int frk_builder_finish(frk_builder* b, frk_table** out); /* the builder stays owned by the caller */
+
+/* Offline index writer (the builder's output, indexer/builder/fr_builder.cpp and
+ * builder/segment/{vector,inverted,idmapping}_builder.cpp): writes <dir>/poissonengine_<version>.meta,
+ * id/ *.idm, and per field one .vector or .posting file per segment -- the directory
+ * frk_table_load reads. Host only. Vectors: rows [n_docs][dim] FP32; file_type
+ * FRK_VECTOR_FILE_FP16_ZN is the reference's FP16 ZN body (split_doc_num_zn docs per block),
+ * FRK_VECTOR_FILE_FP32_ROWS keeps FP32 rows. Postings: CSR over tokens, any id order, duplicates
+ * dropped; token t's text is terms[t] when terms is non-null, else the decimal tags[t]. */
+#define FRK_VECTOR_FILE_FP16_ZN 0u
+#define FRK_VECTOR_FILE_FP32_ROWS 1u
+typedef struct frk_index_vector_field {
+    const char* name;
+    uint32_t dim;
+    uint32_t file_type;
+    const float* rows;
+} frk_index_vector_field;
+typedef struct frk_index_posting_field {
+    const char* name;
+    int32_t is_matrix;
+    float density_threshold;
+    int32_t packed;
+    uint64_t n_tokens;
+    const uint64_t* tags;
+    const char* const* terms;
+    const uint64_t* offsets; /* n_tokens + 1 */
+    const uint32_t* doc_ids;
+} frk_index_posting_field;
+int frk_write_index(const char* dir, uint32_t version, uint32_t n_docs, uint32_t docs_per_segment,
+                    uint32_t split_doc_num_zn, uint32_t n_vector_fields, const frk_index_vector_field* vector_fields,
+                    uint32_t n_posting_fields, const frk_index_posting_field* posting_fields,
+                    const uint64_t* gdocids /* n_docs ids, or null for 0..n_docs-1 */);
+
+/* FP32 -> FP16 as the builder stores vectors (core/number/trans_number.cpp:38-82): round to nearest
+ * even, magnitudes below 2^-14 to signed zero, finite overflow to infinity, infinity and NaN to NaN. */
+uint16_t frk_f32_to_f16(float f);
 
 /* Host codecs of the index format: the token hash (core/hash/hash.cpp:10-16) and the posting
  * serializer (format/posting_serializer.cpp:81-211; SPARSE_IDLIST keeps the given id order).

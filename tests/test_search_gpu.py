@@ -166,3 +166,29 @@ def test_low_precision_with_fp32_rescoring(search_mod, dtype, min_recall):
         assert np.all(np.abs(r[1][q, :r[2][q]] - exact) <= 1e-5 * np.abs(docs[loc] * qs[q]).sum(axis=1) + 1e-30)
     print("dtype %d recall@100 %.4f" % (dtype, hits / tot), flush=True)
     assert hits / tot >= min_recall
+
+
+@pytest.mark.parametrize("dim", [24, 72, 96, 100, 192, 384, 768])
+def test_any_dimension(search_mod, dim):
+    """Dims that are not a compiled chunk count are zero-padded to the next one (12- and 24-chunk
+    shapes cover 96/192/384/768-dim stores without padding): plain FP32 against the CPU truth, BF16 and
+    FP8 with FP32 rescoring against its recall, over 100 queries (4 row blocks per workgroup where the
+    query tile fits)."""
+    from hybridsearchsystem_amd import kernels
+    n, n_tags, nq = 8000 + 45, 20, 100
+    print("dim %d: store dims %s" % (dim, [kernels.padded_dim(dim, d) for d in range(3)]), flush=True)
+    docs, dens, off, ids, words = _corpus(40 + dim, n, dim, n_tags)
+    qs = synth.mixture_queries(41 + dim, nq, dim, 32, corpus_seed=40 + dim)
+    filters, _ = synth.random_filters(42 + dim, dens, nq - 1, depth=2, sel=(0.01, 0.9))
+    filters = [None] + filters
+    ks = np.random.default_rng(dim).choice([1, 10, 50], size=nq).astype(np.uint32)
+    t = search_mod.Table.build(docs, off, ids, docs_per_segment=4096, dtypes=(1, 2))
+    masks, *truth = _truth(docs, words, qs, filters, ks)
+    r = t.search(qs, filters, ks)
+    assert _check(docs, qs, masks, ks, r[0], r[1], r[2], truth) <= 2
+    ids32, _, cnt32 = truth
+    for dtype, min_recall in ((1, 0.99), (2, 0.95)):
+        lo = t.search(qs, filters, ks, dtype=dtype, rescore_factor=4)
+        hits = sum(len(set(ids32[q, :cnt32[q]].tolist()) & set(lo[0][q, :lo[2][q]].tolist())) for q in range(nq))
+        assert hits / max(1, int(cnt32.sum())) >= min_recall, (dtype, hits, int(cnt32.sum()))
+    t.free()
