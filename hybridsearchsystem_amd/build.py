@@ -23,6 +23,10 @@ COMMON = ["-O3", "-fPIC", "-std=c++17", "-Wall", "-Wno-unused-function", "-I" + 
 # Packed FP32 ops are kept out of generated code (device-only flag; the host pass ignores it).
 DEVICE = ["--offload-arch=" + ARCH, "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
           "-munsafe-fp-atomics"]
+# The scorers keep their MFMA accumulators in ordinary VGPRs: the epilogue reads every accumulator
+# once per doc block, and from AGPRs each read would be an extra copy instruction.
+PER_FILE = {name: ["-mllvm", "-amdgpu-mfma-vgpr-form"]
+            for name in ("frk_score_f32.hip", "frk_score_bf16.hip", "frk_score_fp8.hip")}
 
 
 def _sources():
@@ -40,7 +44,8 @@ def _compile(src, is_hip):
     obj = _obj(src)
     # host-only C++ still goes through hipcc (HIP runtime headers); the arch flag keeps it from
     # compiling for a default device target
-    cmd = [HIPCC] + COMMON + (["-x", "hip"] + DEVICE if is_hip else ["--offload-arch=" + ARCH]) + ["-c", src, "-o", obj]
+    cmd = [HIPCC] + COMMON + (["-x", "hip"] + DEVICE if is_hip else ["--offload-arch=" + ARCH])
+    cmd += PER_FILE.get(os.path.basename(src), []) + ["-c", src, "-o", obj]
     p = subprocess.run(cmd, capture_output=True, text=True)
     err = "\n".join(l for l in p.stderr.splitlines() if "packed-fp32-ops" not in l and l.strip())
     return src, p.returncode, err

@@ -380,7 +380,7 @@ bool FullRecallSearcher::RunDevice(const VectorFieldData& vf, std::vector<AggrDe
     const size_t cFilter = 0, cCand = Q, cStatus = 2 * Q, cOut = 2 * Q + 64, cLp = 3 * Q + 64;
     const size_t nCounters = 4 * Q + 64;
     float* dKth = nullptr;
-    const uint32_t qStride = (Q + 31) / 32 * 32;  // block-major masks: one row of qStride u32 per doc block
+    const uint32_t qStride = (Q + 31) / 32 * 32;  // per-doc filter words: qStride u32 per doc block
     if (!ws->bitmaps.Ensure(sizeof(uint64_t) * Q * maskStride) || !ws->bmasks.Ensure(sizeof(uint32_t) * qStride * nBlocks) || !ws->counters.Ensure(sizeof(uint32_t) * nCounters + 4 * Q) ||
         !ws->cand.Ensure(sizeof(uint64_t) * Q * cap) || !ws->out.Ensure(sizeof(uint64_t) * Q * kmax) ||
         (useProbe && !ws->probe.Ensure(sizeof(float) * Q * probeBlocks)) ||

@@ -84,4 +84,17 @@ __host__ __device__ inline uint8_t F32ToE4M3(float f) {
     return static_cast<uint8_t>(sign | (be << 3) | m);
 }
 
+// FP8 scale of a vector whose largest magnitude is mx: the smallest power of two s with mx / s <= 448
+// (E4M3's largest finite value), clamped to [2^-126, 2^127]; 1 for a zero vector. A power of two is an
+// E8M0 exponent, which the scaled FP8 MFMA applies per row and column by itself.
+__host__ __device__ inline float Fp8Scale(float mx) {
+    if (!(mx > 0.0f)) return 1.0f;
+    if (!(mx <= 3.402823466e38f)) return ldexpf(1.0f, 127);
+    int E;
+    const float m = frexpf(mx, &E);  // mx = m * 2^E, m in [0.5, 1); 448 = 0.875 * 2^9
+    int e = m <= 0.875f ? E - 9 : E - 8;
+    e = e < -126 ? -126 : (e > 127 ? 127 : e);
+    return ldexpf(1.0f, e);
+}
+
 }  // namespace frk

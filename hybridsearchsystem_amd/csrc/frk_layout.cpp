@@ -29,7 +29,7 @@ void TileRange(const float* src, uint64_t n_docs, uint32_t dim, uint32_t dtype, 
             if (dtype == FRK_DTYPE_FP8E4M3) {
                 float mx = 0.0f;
                 for (uint32_t d = 0; d < dim; d++) mx = std::max(mx, std::fabs(row[d]));
-                const float sc = mx > 0.0f ? mx / 448.0f : 1.0f;
+                const float sc = frk::Fp8Scale(mx);
                 if (n < n_docs) scales[n] = sc;
                 inv = 1.0f / sc;
             }

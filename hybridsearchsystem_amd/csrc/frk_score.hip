@@ -53,8 +53,8 @@ int Prepare(const frk_doc_store* store, const float* d_queries, uint32_t n_queri
                  (unsigned long long)block_step, (unsigned long long)block_count, (unsigned long long)n_blocks);
         return FRK_ERR_INVALID;
     }
-    if (d_bmask && bmask_stride < n_queries) {
-        SetError("%s: block mask stride %u < %u queries", who, bmask_stride, n_queries);
+    if (d_bmask && (bmask_stride < n_queries || bmask_stride % 32 != 0)) {
+        SetError("%s: filter word stride %u must be a multiple of 32 and >= %u queries", who, bmask_stride, n_queries);
         return FRK_ERR_CAPACITY;
     }
     if (n_queries > 65535u * 32u) {
@@ -104,26 +104,40 @@ int Run(const frk_doc_store* store, const ScoreLaunch& L, bool block_max) {
 
 // Query-major filter words -> block-major 32-bit masks, through a 64 x 64 LDS tile so both the reads
 // (along a query's words) and the writes (along a block's queries) are coalesced.
+// One thread per 32 x 32 bit tile (doc block b, query group g): it reads the 32-bit halves of the 32
+// queries' filter words for the block, transposes the bit matrix in registers (five rounds of
+// off-diagonal block swaps: 16-, 8-, 4-, 2- and 1-bit blocks) and writes the block's 32 per-doc words
+// of query bits with four 16-byte stores. Consecutive threads take consecutive blocks, so each load
+// instruction reads one contiguous run of filter words.
 __global__ __launch_bounds__(256) void MasksToBlocksKernel(const uint64_t* __restrict__ masks, uint64_t stride,
-                                                           uint32_t n_queries, uint64_t n_words,
+                                                           uint32_t n_queries, uint64_t n_blocks,
                                                            uint32_t* __restrict__ out, uint32_t q_stride) {
-    __shared__ uint64_t tile[64][65];
-    const uint64_t w0 = static_cast<uint64_t>(blockIdx.x) * 64;
-    const uint32_t qb = blockIdx.y * 64;
-    for (int e = threadIdx.x; e < 64 * 64; e += 256) {
-        const int qi = e / 64, wi = e % 64;
-        const uint32_t q = qb + qi;
-        const uint64_t w = w0 + wi;
-        tile[qi][wi] = (q < n_queries && w < n_words) ? masks[static_cast<uint64_t>(q) * stride + w] : 0;
+    const uint64_t b = static_cast<uint64_t>(blockIdx.x) * 256 + threadIdx.x;
+    const uint32_t g = blockIdx.y;
+    if (b >= n_blocks) return;
+    uint32_t w[32];
+#pragma unroll
+    for (int x = 0; x < 32; x++) {
+        const uint32_t q = 32 * g + x;
+        w[x] = q < n_queries ? static_cast<uint32_t>(masks[static_cast<uint64_t>(q) * stride + b / 2] >> (32 * (b & 1)))
+                             : 0u;
     }
-    __syncthreads();
-    for (int e = threadIdx.x; e < 128 * 64; e += 256) {
-        const int bi = e / 64, qi = e % 64;  // block = 2 w + half
-        const uint64_t blk = 2 * w0 + bi;
-        const uint32_t q = qb + qi;
-        if (blk < 2 * n_words && q < q_stride)
-            out[blk * q_stride + q] = static_cast<uint32_t>(tile[qi][bi >> 1] >> (32 * (bi & 1)));
+    // after the round of block size s, rows k and k + s (k with bit s clear) have traded the high
+    // s-bit fields of row k for the low s-bit fields of row k + s in every 2s-bit group
+#pragma unroll
+    for (int s = 16; s >= 1; s >>= 1) {
+        const uint32_t m = s == 16 ? 0x0000FFFFu : s == 8 ? 0x00FF00FFu : s == 4 ? 0x0F0F0F0Fu : s == 2 ? 0x33333333u : 0x55555555u;
+#pragma unroll
+        for (int k = 0; k < 32; k++) {
+            if (k & s) continue;
+            const uint32_t t = ((w[k] >> s) ^ w[k + s]) & m;
+            w[k] ^= t << s;
+            w[k + s] ^= t;
+        }
     }
+    u32x4* dst = reinterpret_cast<u32x4*>(out + b * q_stride + 32 * g);
+#pragma unroll
+    for (int i = 0; i < 8; i++) dst[i] = u32x4{w[4 * i], w[4 * i + 1], w[4 * i + 2], w[4 * i + 3]};
 }
 
 }  // namespace
@@ -131,15 +145,21 @@ __global__ __launch_bounds__(256) void MasksToBlocksKernel(const uint64_t* __res
 
 extern "C" int frk_masks_to_blocks(const uint64_t* d_masks, uint64_t mask_stride, uint32_t n_queries, uint64_t n_docs,
                                    uint32_t* d_block_masks, uint32_t q_stride, frk_stream_t stream) {
-    if (!d_masks || !d_block_masks || q_stride < n_queries) {
-        frk::SetError("frk_masks_to_blocks: bad arguments");
+    if (!d_masks || !d_block_masks || q_stride < n_queries || q_stride % 32 != 0 ||
+        (reinterpret_cast<uintptr_t>(d_block_masks) & 15) != 0) {
+        frk::SetError("frk_masks_to_blocks: bad arguments (q_stride %u must be a multiple of 32 and >= %u queries; "
+                      "the output 16-byte aligned)", q_stride, n_queries);
         return FRK_ERR_INVALID;
     }
-    const uint64_t n_words = (n_docs + 63) / 64;
-    if (n_words == 0 || q_stride == 0) return FRK_OK;
-    dim3 grid((n_words + 63) / 64, (q_stride + 63) / 64);
+    const uint64_t n_blocks = (n_docs + 31) / 32;
+    if (n_blocks == 0 || q_stride == 0) return FRK_OK;
+    if (q_stride / 32 > 65535u) {
+        frk::SetError("frk_masks_to_blocks: %u queries exceed the grid", q_stride);
+        return FRK_ERR_UNSUPPORTED;
+    }
+    dim3 grid(static_cast<uint32_t>((n_blocks + 255) / 256), q_stride / 32);
     hipLaunchKernelGGL(frk::MasksToBlocksKernel, grid, dim3(256), 0, static_cast<hipStream_t>(stream), d_masks,
-                       mask_stride, n_queries, n_words, d_block_masks, q_stride);
+                       mask_stride, n_queries, n_blocks, d_block_masks, q_stride);
     FRK_HIP_CHECK(hipGetLastError());
     return FRK_OK;
 }

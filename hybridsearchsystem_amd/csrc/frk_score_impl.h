@@ -4,10 +4,11 @@
 // The reference computes S = Q x D^T on the Cube unit into a full [Q][docs] FP32 matrix
 // (engine/retrieval/device/scorer/kernel_vector_mmad_op.h:21-199) and a second kernel re-reads it with
 // the filter bitmap to compact survivors (device/aggregator/kernel_aggregator.h:22-182). Here one pass
-// does both: a wave computes a 32-doc block against the whole query tile held in LDS, then each
-// accumulator row is tested against (mask bit, floor), ballot-ed, and the passing (score, doc) pairs
-// are appended to per-query candidate lists at offsets from one atomic per (wave, query) and the
-// prefix count of the ballot.
+// does both: a wave computes a 32-doc block against the whole query tile held in LDS, its MFMA chain
+// starting from 0 for (query, doc) pairs that pass the filter and from NaN for the others, then each
+// accumulator row is tested against its floor, ballot-ed, and the passing (score, doc) pairs are
+// appended to per-query candidate lists at offsets from one atomic per (wave, query) and the prefix
+// count of the ballot.
 //
 // Layout (frk_capi.h, "Doc store"): a 32-doc x 16-byte-per-lane chunk tile is exactly one wave load of
 // 1 KiB and exactly the B operand of one MFMA step group:
@@ -74,14 +75,15 @@ __device__ __forceinline__ void MfmaChunk<FRK_DTYPE_BF16>(const u32x4& a, const 
                                                   0, 0, 0);
 }
 
+// sa, sb: E8M0 scale exponents (2^(e - 127)) of this lane's query row and doc column
 __device__ __forceinline__ void MfmaChunkPairFp8(const u32x4& a0, const u32x4& a1, const u32x4& b0, const u32x4& b1,
-                                        floatx16& acc) {
+                                        floatx16& acc, int sa, int sb) {
     intx8 av = {static_cast<int>(a0[0]), static_cast<int>(a0[1]), static_cast<int>(a0[2]), static_cast<int>(a0[3]),
                 static_cast<int>(a1[0]), static_cast<int>(a1[1]), static_cast<int>(a1[2]), static_cast<int>(a1[3])};
     intx8 bv = {static_cast<int>(b0[0]), static_cast<int>(b0[1]), static_cast<int>(b0[2]), static_cast<int>(b0[3]),
                 static_cast<int>(b1[0]), static_cast<int>(b1[1]), static_cast<int>(b1[2]), static_cast<int>(b1[3])};
-    // formats: 0 = FP8 E4M3 for both operands; scales 127 = 2^0
-    acc = __builtin_amdgcn_mfma_scale_f32_32x32x64_f8f6f4(av, bv, acc, 0, 0, 0, 127, 0, 127);
+    // formats: 0 = FP8 E4M3 for both operands; op_sel 0 = byte 0 of each scale register
+    acc = __builtin_amdgcn_mfma_scale_f32_32x32x64_f8f6f4(av, bv, acc, 0, 0, 0, sa, 0, sb);
 }
 
 struct ScoreArgs {
@@ -92,7 +94,8 @@ struct ScoreArgs {
     uint32_t n_queries;
     uint32_t dim;  // padded dim of the store
     uint32_t q_dim;  // row length of the query matrix (== dim)
-    const uint32_t* bmask;    // block-major filter masks: bmask[block * bmask_stride + q], bit j = doc 32 block + j
+    const uint32_t* bmask;    // per-doc filter words: bmask[block * bmask_stride + 32 g + j] bit x = query 32 g + x
+                              // passes doc 32 block + j
     uint32_t bmask_stride;    // >= n_queries; 0 = no filter (bmask then points at one all-pass word)
     const float* floor;
     uint64_t block_begin, block_step, block_count;
@@ -103,7 +106,7 @@ struct ScoreArgs {
 };
 
 // Writes the query tile into LDS in the doc tiling, converted to the store's element type; for fp8 the
-// per-query scale (max|q| / 448) goes to q_scale_lds.
+// per-query power-of-two scale (Fp8Scale of max|q|) goes to q_scale_lds.
 template <int DT, int CH, int RB>
 __device__ __forceinline__ void StageQueries(const ScoreArgs& A, u32x4* a_lds, float* q_scale_lds, uint32_t q0) {
     constexpr int E = Traits<DT>::kElemsPerLane;
@@ -113,7 +116,7 @@ __device__ __forceinline__ void StageQueries(const ScoreArgs& A, u32x4* a_lds, f
             float mx = 0.0f;
             if (q < A.n_queries)
                 for (uint32_t d = 0; d < A.dim; d++) mx = fmaxf(mx, fabsf(A.queries[static_cast<uint64_t>(q) * A.q_dim + d]));
-            q_scale_lds[r] = mx > 0.0f ? mx / 448.0f : 1.0f;
+            q_scale_lds[r] = Fp8Scale(mx);
         }
         __syncthreads();
     }
@@ -175,9 +178,10 @@ struct GroupShape {
     static_assert(kUnroll % kGroups == 0 && kUnroll % kRing == 0, "loop body must hold whole blocks and rings");
 };
 
+// qexp: per row block the E8M0 exponent of this lane's query row scale; dexp: the doc's (FP8 only).
 template <int DT, int CH, int RB>
 __device__ __forceinline__ void ComputeGroup(const u32x4* a_lds, int g, const u32x4 (&buf)[GroupShape<CH>::G],
-                                    floatx16 (&acc)[RB]) {
+                                    floatx16 (&acc)[RB], const int (&qexp)[RB], int dexp) {
     constexpr int G = GroupShape<CH>::G;
     const int lane = threadIdx.x & 63;
     if constexpr (DT == FRK_DTYPE_FP8E4M3) {
@@ -187,7 +191,7 @@ __device__ __forceinline__ void ComputeGroup(const u32x4* a_lds, int g, const u3
             for (int rb = 0; rb < RB; rb++) {
                 const u32x4 a0 = a_lds[(rb * CH + g * G + c) * 64 + lane];
                 const u32x4 a1 = a_lds[(rb * CH + g * G + c + 1) * 64 + lane];
-                MfmaChunkPairFp8(a0, a1, buf[c], buf[c + 1], acc[rb]);
+                MfmaChunkPairFp8(a0, a1, buf[c], buf[c + 1], acc[rb], qexp[rb], dexp);
             }
         }
     } else {
@@ -250,60 +254,75 @@ __device__ __forceinline__ void FlushQueue(const ScoreArgs& A, WaveQueue& Qu) {
     Qu.nr = 0;
 }
 
-// The block's filter masks for this wave's row blocks: lane l loads query row (l % 32) of each row block
-// (one coalesced 128-byte load per row block) a few steps before the block's epilogue; the epilogue
-// stages them through LDS so each lane can read the 16 rows it holds.
-template <int RB>
-__device__ __forceinline__ void LoadMasks(const ScoreArgs& A, uint32_t q0, uint64_t blk, uint32_t (&m)[RB]) {
+// The block's filter words for this wave's row blocks: for row block rb, lane l loads the word of doc
+// (l % 32) whose bit x says whether query q0 + 32 rb + x passes it (one coalesced 128-byte load per row
+// block), together with the block's first group of doc tiles; for FP8 also the doc's scale.
+template <int DT, int RB>
+__device__ __forceinline__ void LoadMasks(const ScoreArgs& A, uint32_t q0, uint64_t blk, uint32_t (&m)[RB],
+                                          float& dscale) {
     const int lane = threadIdx.x & 63;
+    if constexpr (DT == FRK_DTYPE_FP8E4M3) {
+        // the doc's scale (a doc past the end reads the last doc's; its scores are NaN anyway)
+        const uint64_t doc = blk * 32 + (lane & 31);
+        dscale = A.doc_scale[doc < A.n_docs ? doc : A.n_docs - 1];
+    }
     // bmask_stride 0: no filter, bmask is one all-pass word (no branch, so the loads stay in
     // straight-line code)
     const uint32_t* base = A.bmask + blk * A.bmask_stride;
 #pragma unroll
     for (int rb = 0; rb < RB; rb++) {
-        const uint32_t q = q0 + 32 * rb + (lane & 31);
-        const uint32_t off = A.bmask_stride ? (q < A.n_queries ? q : A.n_queries - 1) : 0u;
+        const uint32_t r0 = q0 + 32 * rb;
+        const uint32_t off = A.bmask_stride && r0 < A.n_queries ? r0 + (lane & 31) : 0u;
         const uint32_t v = base[off];
-        m[rb] = q < A.n_queries ? v : 0u;
+        m[rb] = r0 < A.n_queries ? v : 0u;
     }
 }
 
+// The filter folded into the accumulators: each block's MFMA chain starts from 0 where the doc passes
+// the row's query and from NaN where it does not (or the doc is past the end), so a filtered score is
+// NaN, which no floor comparison accepts and no max picks. Lane half h holds rows 8 g + 4 h + x of
+// accumulator register 4 g + x; the all-ones word (a NaN) comes from sign-extending the doc's inverted
+// filter bit, one bit-field extract per register.
 template <int RB>
-__device__ __forceinline__ void StageMasks(const uint32_t (&m)[RB], uint32_t* mask_lds) {
-    const int lane = threadIdx.x & 63;
+__device__ __forceinline__ void InitBias(const uint32_t (&m)[RB], bool doc_ok, floatx16 (&acc)[RB]) {
+    const uint32_t half = (threadIdx.x & 63) >> 5;
 #pragma unroll
-    for (int rb = 0; rb < RB; rb++)
-        if (lane < 32) mask_lds[32 * rb + lane] = m[rb];
-    WaveSync();
+    for (int rb = 0; rb < RB; rb++) {
+        const uint32_t nw = doc_ok ? ~(m[rb] >> (4 * half)) : 0xFFFFFFFFu;
+#pragma unroll
+        for (int g = 0; g < 4; g++)
+#pragma unroll
+            for (int x = 0; x < 4; x++)
+                acc[rb][4 * g + x] = __int_as_float(-static_cast<int32_t>((nw >> (8 * g + x)) & 1u));
+    }
 }
 
 // One row block of the epilogue; rb is a template parameter so the accumulator is always indexed
 // statically (a runtime row-block index would send it to scratch).
-template <int DT, int RB, int rb>
-__device__ __forceinline__ void EpilogueRowBlock(const ScoreArgs& A, const float* q_scale_lds, const float* floor_lds,
-                                        const uint32_t* mask_lds, uint32_t q0, uint64_t doc, bool doc_ok, float dscale,
+template <int RB, int rb>
+__device__ __forceinline__ void EpilogueRowBlock(const ScoreArgs& A, const float* floor_lds, uint32_t q0, uint64_t doc,
                                         const floatx16& accr, WaveQueue& Qu) {
     const int lane = threadIdx.x & 63;
-    const uint32_t j = lane & 31;
     const uint32_t half = lane >> 5;
     {
-        uint32_t hits = 0;
-        float sv[16];
+        float sv[16], fl[16];
+        uint64_t any = 0;  // lanes with a passing row: one compare per score (the filter is in the score)
 #pragma unroll
         for (int g = 0; g < 4; g++) {
             const int row = 32 * rb + 8 * g + 4 * half;
             const floatx4 f4 = *reinterpret_cast<const floatx4*>(floor_lds + row);
-            const u32x4 m4 = *reinterpret_cast<const u32x4*>(mask_lds + row);
 #pragma unroll
             for (int x = 0; x < 4; x++) {
-                float v = accr[4 * g + x];
-                if constexpr (DT == FRK_DTYPE_FP8E4M3) v *= dscale * q_scale_lds[row + x];
+                const float v = accr[4 * g + x];
                 sv[4 * g + x] = v;
-                hits |= static_cast<uint32_t>(v >= f4[x] && ((m4[x] >> j) & 1u)) << (4 * g + x);
+                fl[4 * g + x] = f4[x];
+                any |= __ballot(v >= f4[x]);
             }
         }
-        if (!doc_ok) hits = 0;
-        if (__ballot(hits != 0) == 0) return;
+        if (any == 0) return;
+        uint32_t hits = 0;
+#pragma unroll
+        for (int r = 0; r < 16; r++) hits |= static_cast<uint32_t>(sv[r] >= fl[r]) << r;
         // groups of kAppendRows rows: count a group's keys and runs first, flush if they would not fit,
         // then append (so the unrolled append loop itself never flushes). A group holds at most
         // 64 kAppendRows keys and 2 kAppendRows runs, which an empty queue always takes.
@@ -342,25 +361,19 @@ __device__ __forceinline__ void EpilogueRowBlock(const ScoreArgs& A, const float
     }
 }
 
-template <int DT, int RB, int... I>
-__device__ __forceinline__ void EpilogueAll(const ScoreArgs& A, const float* q_scale_lds, const float* floor_lds,
-                                   const uint32_t* mask_lds, uint32_t q0, uint64_t doc, bool doc_ok, float dscale,
+template <int RB, int... I>
+__device__ __forceinline__ void EpilogueAll(const ScoreArgs& A, const float* floor_lds, uint32_t q0, uint64_t doc,
                                    const floatx16 (&acc)[RB], WaveQueue& Qu, std::integer_sequence<int, I...>) {
-    (EpilogueRowBlock<DT, RB, I>(A, q_scale_lds, floor_lds, mask_lds, q0, doc, doc_ok, dscale, acc[I], Qu), ...);
+    (EpilogueRowBlock<RB, I>(A, floor_lds, q0, doc, acc[I], Qu), ...);
 }
 
-// Epilogue of one finished 32-doc block: every accumulator row against (floor, filter bit); survivors
-// go to the wave queue.
-template <int DT, int RB>
-__device__ __forceinline__ void Epilogue(const ScoreArgs& A, const float* q_scale_lds, const float* floor_lds,
-                                const uint32_t* mask_lds, uint32_t q0, uint64_t blk, const floatx16 (&acc)[RB],
-                                WaveQueue& Qu) {
+// Epilogue of one finished 32-doc block: every accumulator row against its floor (filtered docs and
+// docs past the end hold NaN); survivors go to the wave queue.
+template <int RB>
+__device__ __forceinline__ void Epilogue(const ScoreArgs& A, const float* floor_lds, uint32_t q0, uint64_t blk,
+                                const floatx16 (&acc)[RB], WaveQueue& Qu) {
     const uint64_t doc = blk * 32 + (threadIdx.x & 31);
-    const bool doc_ok = doc < A.n_docs;
-    float dscale = 1.0f;
-    if constexpr (DT == FRK_DTYPE_FP8E4M3) dscale = doc_ok ? A.doc_scale[doc] : 0.0f;
-    EpilogueAll<DT, RB>(A, q_scale_lds, floor_lds, mask_lds, q0, doc, doc_ok, dscale, acc, Qu,
-                        std::make_integer_sequence<int, RB>{});
+    EpilogueAll<RB>(A, floor_lds, q0, doc, acc, Qu, std::make_integer_sequence<int, RB>{});
 }
 
 // One stage of the block-max butterfly over lane bit B with N rows left per lane (M = array size):
@@ -388,39 +401,27 @@ __device__ __forceinline__ void BlockMaxStages(float (&v)[M], uint32_t j, uint32
 }
 
 // Probe epilogue: per (query, block) the largest score among the block's docs that pass the filter
-// (-inf if none), written to blockmax[q * A.cap + block index] -- no atomics, no compaction.
+// (-inf if none), written to blockmax[q * A.cap + block index] -- no atomics, no compaction. Filtered
+// docs hold NaN, which the maximum (IEEE maxNum) passes over; a row with no passing doc ends as NaN
+// and is written as -inf.
 // The max over the 32 docs (lanes j of each half) is a butterfly transpose-reduce: at each of the five
 // lane-bit stages a lane keeps half of its rows and trades the other half with its partner, so the
 // 16 RB rows a lane holds shrink to 16 RB / 32 after 16 RB - 1 exchanges in total (instead of five
 // exchanges per row). After it, lane j of half h holds the rows of flat accumulator indices 2j, 2j+1
 // (RB = 4), or the row of index j (RB = 2); with RB = 1 the last stage is a plain exchange of the
 // single remaining row, index j / 2, which lanes j and j ^ 1 then share.
-template <int DT, int RB>
-__device__ __forceinline__ void EpilogueBlockMax(const ScoreArgs& A, const float* q_scale_lds, const uint32_t* mask_lds,
-                                        uint32_t q0, uint64_t blk, uint64_t step_index, const floatx16 (&acc)[RB]) {
+template <int RB>
+__device__ __forceinline__ void EpilogueBlockMax(const ScoreArgs& A, uint32_t q0, uint64_t step_index,
+                                        const floatx16 (&acc)[RB]) {
     constexpr int N = 16 * RB;
     const int lane = threadIdx.x & 63;
     const uint32_t j = lane & 31;
-    const uint64_t doc = blk * 32 + j;
-    const bool doc_ok = doc < A.n_docs;
-    float dscale = 1.0f;
-    if constexpr (DT == FRK_DTYPE_FP8E4M3) dscale = doc_ok ? A.doc_scale[doc] : 0.0f;
     const uint32_t half = lane >> 5;
     float v[N];
 #pragma unroll
-    for (int rb = 0; rb < RB; rb++) {
+    for (int rb = 0; rb < RB; rb++)
 #pragma unroll
-        for (int g = 0; g < 4; g++) {
-            const int row = 32 * rb + 8 * g + 4 * half;
-            const u32x4 m4 = *reinterpret_cast<const u32x4*>(mask_lds + row);
-#pragma unroll
-            for (int x = 0; x < 4; x++) {
-                float s = acc[rb][4 * g + x];
-                if constexpr (DT == FRK_DTYPE_FP8E4M3) s *= dscale * q_scale_lds[row + x];
-                v[16 * rb + 4 * g + x] = (doc_ok && ((m4[x] >> j) & 1u)) ? s : -INFINITY;
-            }
-        }
-    }
+        for (int r = 0; r < 16; r++) v[16 * rb + r] = acc[rb][r];
     uint32_t base = 0;  // flat index of v[0] after the halving stages
     BlockMaxStages<N, N, 16>(v, j, base);
     constexpr int n = N >= 32 ? N / 32 : 1;  // rows left per lane
@@ -433,7 +434,7 @@ __device__ __forceinline__ void EpilogueBlockMax(const ScoreArgs& A, const float
         const uint32_t r = base + e;
         const uint32_t row = 32 * (r >> 4) + 8 * ((r >> 2) & 3) + 4 * half + (r & 3);
         const uint32_t q = q0 + row;
-        if (q < A.n_queries) out[static_cast<uint64_t>(q) * A.cap + step_index] = v[e];
+        if (q < A.n_queries) out[static_cast<uint64_t>(q) * A.cap + step_index] = v[e] == v[e] ? v[e] : -INFINITY;
     }
 }
 
@@ -445,8 +446,7 @@ __global__ __launch_bounds__(kThreads, FRK_SCORE_MIN_BLOCKS) void ScoreKernel(Sc
     u32x4* a_lds = smem;                                               // [RB][CH][64]
     float* q_scale_lds = reinterpret_cast<float*>(smem + RB * CH * 64);  // [32 * RB]
     float* floor_lds = q_scale_lds + 32 * RB;                          // [32 * RB]
-    uint32_t* masks_lds = reinterpret_cast<uint32_t*>(floor_lds + 32 * RB);  // [waves][32 * RB]
-    uint64_t* queue_lds = reinterpret_cast<uint64_t*>(masks_lds + kWaves * 32 * RB);  // [waves][keys + runs]
+    uint64_t* queue_lds = reinterpret_cast<uint64_t*>(floor_lds + 32 * RB);  // [waves][keys + runs]
     // Workgroup -> (doc range x, query group): with several query groups every doc range is read once
     // per group, so the groups of one range go to the same XCD (workgroups are dealt to the 8 XCDs
     // round-robin by linear id) and run back to back there, sharing that XCD's L2 instead of each
@@ -473,7 +473,6 @@ __global__ __launch_bounds__(kThreads, FRK_SCORE_MIN_BLOCKS) void ScoreKernel(Sc
 
     const int lane = threadIdx.x & 63;
     const int wave = threadIdx.x >> 6;
-    uint32_t* mask_lds = masks_lds + wave * 32 * RB;
     WaveQueue Qu;
     Qu.keys = queue_lds + wave * (kQueueKeys + kQueueRuns);
     Qu.runs = Qu.keys + kQueueKeys;
@@ -488,7 +487,15 @@ __global__ __launch_bounds__(kThreads, FRK_SCORE_MIN_BLOCKS) void ScoreKernel(Sc
 
     auto block_index = [&](uint64_t step) -> uint64_t { return i_lo + wave + (step / NG) * kWaves; };
     u32x4 ring[kRing][G];
-    uint32_t mring[kRing][RB];  // mask words of the block whose last group sits in the same slot
+    uint32_t mring[kRing][RB];  // filter words of the block whose first group sits in the same slot
+    float sring[kRing];         // FP8: that block's doc scale
+    int qexp[RB] = {};          // FP8: E8M0 exponent of this lane's query row scale, per row block
+    if constexpr (DT == FRK_DTYPE_FP8E4M3) {
+#pragma unroll
+        for (int rb = 0; rb < RB; rb++)
+            qexp[rb] = static_cast<int>((__float_as_uint(q_scale_lds[32 * rb + (lane & 31)]) >> 23) & 0xFFu);
+    }
+    int dexp = 127;
     // load step `step` (group index GL, a compile-time constant) into ring slot SL
     auto load = [&](auto slot_c, auto group_c, uint64_t step) __attribute__((always_inline)) {
         constexpr int SL = decltype(slot_c)::value;
@@ -498,7 +505,7 @@ __global__ __launch_bounds__(kThreads, FRK_SCORE_MIN_BLOCKS) void ScoreKernel(Sc
         const u32x4* p = A.tiles + (blk * CH + static_cast<uint64_t>(GL) * G) * 64 + lane;
 #pragma unroll
         for (int c = 0; c < G; c++) ring[SL][c] = p[c * 64];
-        if constexpr (GL == NG - 1) LoadMasks<RB>(A, q0, blk, mring[SL]);
+        if constexpr (GL == 0) LoadMasks<DT, RB>(A, q0, blk, mring[SL], sring[SL]);
     };
     // the first kRing - 1 steps, in step order (each wait below counts on that order)
     load(std::integral_constant<int, 0>{}, std::integral_constant<int, 0 % NG>{}, 0);
@@ -507,10 +514,7 @@ __global__ __launch_bounds__(kThreads, FRK_SCORE_MIN_BLOCKS) void ScoreKernel(Sc
     __builtin_amdgcn_sched_barrier(0);
     load(std::integral_constant<int, 2>{}, std::integral_constant<int, 2 % NG>{}, 2);
     __builtin_amdgcn_sched_barrier(0);
-    floatx16 acc[RB];
-#pragma unroll
-    for (int rb = 0; rb < RB; rb++)
-        for (int r = 0; r < 16; r++) acc[rb][r] = 0.0f;
+    floatx16 acc[RB];  // set from the filter words at the start of every block
     // step s of the loop body (body starts at a block boundary `base`)
     auto step_fn = [&](auto s_c, uint64_t base) __attribute__((always_inline)) {
         constexpr int s = decltype(s_c)::value;
@@ -522,19 +526,19 @@ __global__ __launch_bounds__(kThreads, FRK_SCORE_MIN_BLOCKS) void ScoreKernel(Sc
         // keep the prefetch at the head of the step: the scheduler would otherwise sink the loads
         // towards their use and shrink the distance the ring is there to provide
         __builtin_amdgcn_sched_barrier(0);
-        ComputeGroup<DT, CH, RB>(a_lds, g, ring[S], acc);
+        if constexpr (g == 0) {
+            const uint64_t blk = A.block_begin + block_index(step) * A.block_step;
+            InitBias<RB>(mring[S], blk * 32 + (lane & 31) < A.n_docs, acc);
+            if constexpr (DT == FRK_DTYPE_FP8E4M3) dexp = static_cast<int>((__float_as_uint(sring[S]) >> 23) & 0xFFu);
+        }
+        ComputeGroup<DT, CH, RB>(a_lds, g, ring[S], acc, qexp, dexp);
         if constexpr (g == NG - 1) {
             const uint64_t i = block_index(step);
             const uint64_t blk = A.block_begin + i * A.block_step;
-            StageMasks<RB>(mring[S], mask_lds);
             if constexpr (kBlockMax)
-                EpilogueBlockMax<DT, RB>(A, q_scale_lds, mask_lds, q0, blk, i, acc);
+                EpilogueBlockMax<RB>(A, q0, i, acc);
             else
-                Epilogue<DT, RB>(A, q_scale_lds, floor_lds, mask_lds, q0, blk, acc, Qu);
-#pragma unroll
-            for (int rb = 0; rb < RB; rb++)
-#pragma unroll
-                for (int r = 0; r < 16; r++) acc[rb][r] = 0.0f;
+                Epilogue<RB>(A, floor_lds, q0, blk, acc, Qu);
         }
     };
     uint64_t base = 0;
@@ -558,7 +562,7 @@ constexpr uint32_t kScoreShapes[] = {4, 8, 12, 16, 24, 32, 48, 64, 96, 128};
 template <int DT, int CH, int RB, bool kBlockMax>
 int Launch(const ScoreArgs& A, hipStream_t stream, uint32_t grid_x) {
     const size_t lds = static_cast<size_t>(RB) * CH * 64 * sizeof(u32x4) + 2 * 32 * RB * sizeof(float) +
-                       kWaves * 32 * RB * sizeof(uint32_t) + kWaves * (kQueueKeys + kQueueRuns) * sizeof(uint64_t);
+                       kWaves * (kQueueKeys + kQueueRuns) * sizeof(uint64_t);
     const void* fn = reinterpret_cast<const void*>(&ScoreKernel<DT, CH, RB, kBlockMax>);
     static int resident_per_cu = 0;  // workgroups of this shape one CU holds at once
     if (resident_per_cu == 0) {

@@ -14,7 +14,7 @@ __global__ __launch_bounds__(256) void RowScaleKernel(const float* __restrict__ 
     float mx = 0.0f;
     for (uint32_t d = lane; d < dim; d += 64) mx = fmaxf(mx, fabsf(rows[r * dim + d]));
     for (int off = 32; off > 0; off >>= 1) mx = fmaxf(mx, __shfl_xor(mx, off, 64));
-    if (lane == 0) scales[r] = mx > 0.0f ? mx / 448.0f : 1.0f;
+    if (lane == 0) scales[r] = Fp8Scale(mx);
 }
 
 __global__ __launch_bounds__(256) void TileKernel(const float* __restrict__ rows, uint64_t n, uint32_t dim,

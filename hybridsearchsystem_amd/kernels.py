@@ -61,14 +61,17 @@ def masks_to_blocks(d_masks, mask_stride, n_queries, n_docs, d_block_masks, q_st
 
 
 def block_masks_host(bits):
-    """bool [n_queries, n_docs] -> block-major uint32 [n_blocks, n_queries] (bit j = doc 32 b + j)."""
+    """bool [n_queries, n_docs] -> the scorer's per-doc query words, uint32 [n_blocks, round_up(n_queries, 32)]:
+    word [b, 32 g + j] bit x = bits[32 g + x, 32 b + j]."""
     bits = np.asarray(bits, dtype=bool)
     nq, n = bits.shape
     nb = (n + 31) // 32
-    pad = np.zeros((nq, nb * 32), dtype=bool)
-    pad[:, :n] = bits
-    w = (pad.reshape(nq, nb, 32).astype(np.uint64) << np.arange(32, dtype=np.uint64)).sum(axis=2, dtype=np.uint64)
-    return np.ascontiguousarray(w.T.astype(np.uint32))
+    qs = (nq + 31) // 32 * 32
+    pad = np.zeros((qs, nb * 32), dtype=bool)
+    pad[:nq, :n] = bits
+    t = pad.reshape(qs // 32, 32, nb, 32)  # [g, x, b, j]
+    w = (t.astype(np.uint64) << np.arange(32, dtype=np.uint64)[None, :, None, None]).sum(axis=1, dtype=np.uint64)
+    return np.ascontiguousarray(w.transpose(1, 0, 2).reshape(nb, qs).astype(np.uint32))  # [b, g, j]
 
 
 def score_select(store, d_queries, n_queries, d_block_masks, mask_q_stride, d_floor, block_begin, block_step,

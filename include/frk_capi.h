@@ -120,8 +120,10 @@ int frk_tile_docs(const float* src, uint64_t n_docs, uint32_t dim, uint32_t dtyp
 int frk_tile_rows_device(const float* d_rows, uint64_t n_rows, uint32_t dim, uint64_t first_doc, uint32_t dtype,
                          uint32_t dim_pad, void* d_tiles, float* d_scales, frk_stream_t stream);
 
-/* Filter output (query-major u64 words, as frk_filter_eval writes it) -> the block-major 32-bit masks
- * the scorer reads: out[b * q_stride + q] = docs 32b .. 32b+31 of query q (0 for q >= n_queries). */
+/* Filter output (query-major u64 words, as frk_filter_eval writes it) -> the per-doc query words the
+ * scorer reads: out[b * q_stride + 32 g + j] bit x = query 32 g + x passes doc 32 b + j (each 32 x 32
+ * bit tile of queries x docs transposed; bits of queries >= n_queries are 0). q_stride: a multiple of
+ * 32, >= n_queries; d_block_masks 16-byte aligned. */
 int frk_masks_to_blocks(const uint64_t* d_masks, uint64_t mask_stride, uint32_t n_queries, uint64_t n_docs,
                         uint32_t* d_block_masks, uint32_t q_stride, frk_stream_t stream);
 
@@ -131,9 +133,9 @@ int frk_masks_to_blocks(const uint64_t* d_masks, uint64_t mask_stride, uint32_t 
  * per-query candidate lists. Replaces kernel_vector_mmad (scorer/kernel_vector_mmad_op.h:21-199)
  * and Aggregator (aggregator/kernel_aggregator.h:22-182): no score matrix is written.
  *
- * Doc blocks visited: block_begin + i * block_step for i < block_count. A doc n passes if its bit in
- * the block-major filter masks is set (d_block_masks[(n / 32) * mask_q_stride + q], bit n % 32;
- * NULL = every doc; see frk_masks_to_blocks) and S[q][n] >= floor[q] (d_floor NULL = no floor). A
+ * Doc blocks visited: block_begin + i * block_step for i < block_count. A doc n passes if its query
+ * bit is set (d_block_masks[(n / 32) * mask_q_stride + 32 (q / 32) + n % 32], bit q % 32; NULL = every
+ * doc; see frk_masks_to_blocks) and S[q][n] >= floor[q] (d_floor NULL = no floor). A
  * passing doc appends MakeKey(S, n) (score-ordered high word, reversed doc id low word) to
  * cand[q * cand_cap + slot]; counts[q] counts every pass, so counts[q] > cand_cap means the list
  * overflowed and was truncated. d_queries: [n_queries][dim] FP32 row-major.

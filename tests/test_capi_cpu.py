@@ -94,3 +94,39 @@ def test_synthetic_postings_follow_densities():
     for t in range(4):
         d = ids[off[t]:off[t + 1]]
         assert np.all(np.diff(d.astype(np.int64)) > 0) and (len(d) == 0 or d[-1] < 200000)
+
+
+def test_block_mask_layout_is_per_doc_query_words(L):
+    """The scorer's filter words: word [b, 32 g + j] bit x = query 32 g + x passes doc 32 b + j."""
+    from hybridsearchsystem_amd import kernels
+    rng = np.random.default_rng(4)
+    nq, n = 45, 200
+    bits = rng.random((nq, n)) < 0.3
+    w = kernels.block_masks_host(bits)
+    assert w.shape == ((n + 31) // 32, 64) and w.dtype == np.uint32
+    for b in range(w.shape[0]):
+        for g in range(2):
+            for j in range(32):
+                for x in range(32):
+                    q, d = 32 * g + x, 32 * b + j
+                    want = bool(bits[q, d]) if q < nq and d < n else False
+                    assert bool((int(w[b, 32 * g + j]) >> x) & 1) == want
+
+
+def test_fp8_scales_are_powers_of_two(L):
+    """FP8 stores scale each doc by the smallest power of two that brings max|v| to at most 448."""
+    from hybridsearchsystem_amd import kernels
+    rng = np.random.default_rng(5)
+    docs = rng.standard_normal((96, 64)).astype(np.float32) * rng.uniform(1e-3, 1e3, (96, 1)).astype(np.float32)
+    docs[3] = 0.0
+    docs[7, :] = 0.0
+    docs[7, 5] = 448.0   # exactly the largest E4M3 value: scale 1
+    docs[8, :] = 0.0
+    docs[8, 5] = 449.0   # just above: scale 2
+    _, scales, _ = kernels.tile_docs(docs, kernels.FRK_DTYPE_FP8E4M3)
+    mx = np.abs(docs).max(axis=1)
+    e = np.log2(scales.astype(np.float64))
+    assert np.all(e == np.round(e))
+    nz = mx > 0
+    assert np.all(mx[nz] / scales[nz] <= 448.0) and np.all(mx[nz] / scales[nz] > 224.0)
+    assert scales[3] == 1.0 and scales[7] == 1.0 and scales[8] == 2.0

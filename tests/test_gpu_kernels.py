@@ -175,7 +175,7 @@ def _run_topk(frk, docs, qs, masks_words, ks, dtype=0, floor=None, block_step=1,
         frk.masks_to_blocks(d_words, mw, nq, n, d_m, qs_stride)
         bits = np.stack([oracle.bits_from_words(w, n) for w in masks_words])
         want = frk.block_masks_host(bits)
-        assert np.array_equal(d_m.cpu().numpy().view(np.uint32)[:, :nq], want)
+        assert np.array_equal(d_m.cpu().numpy().view(np.uint32), want)
     cap = cap or n
     d_cand = torch.zeros((nq, cap), dtype=torch.int64, device="cuda")
     d_cnt = torch.zeros(nq, dtype=torch.int32, device="cuda")
@@ -235,6 +235,59 @@ def test_fp32_topk_matches_cpu_truth(frk, n, dim, nq):
     _check_against_truth(frk, docs, qs, bits, ks, keys, oc)
 
 
+def _fp8_scale(mx):
+    """Smallest power of two s with mx / s <= 448 (1 for 0): the FP8 stores' per-vector scale."""
+    mx = np.asarray(mx, np.float64)
+    e = np.ceil(np.log2(np.where(mx > 0, mx, 448.0) / 448.0))
+    return np.where(mx > 0, 2.0 ** e, 1.0)
+
+
+def _dequantized(x, dtype):
+    """The values a low-precision store (or staged query tile) holds, widened to float64."""
+    t = torch.from_numpy(np.ascontiguousarray(x, np.float32))
+    if dtype == 1:
+        return t.to(torch.bfloat16).to(torch.float64).numpy()
+    s = _fp8_scale(np.abs(x).max(axis=1))[:, None]
+    return (torch.from_numpy((x / s).astype(np.float32)).to(torch.float8_e4m3fn).to(torch.float64).numpy() * s)
+
+
+@pytest.mark.parametrize("dtype", [1, 2])
+def test_low_precision_scores_match_dequantized_values(frk, dtype):
+    """Every filter-passing doc comes out once with the dot product of the stored (dequantized) query and
+    doc values. Doc and query magnitudes span six decades, so a per-lane scale applied to the wrong row
+    or column of the scaled FP8 MFMA shows; 3,000 docs leave a ragged last block, 70 queries three row
+    blocks of one query group."""
+    print("low precision scores dtype %d" % dtype, flush=True)
+    rng = np.random.default_rng(60 + dtype)
+    n, dim, nq = 3000, 128, 70
+    docs = (rng.standard_normal((n, dim)) * 10.0 ** rng.uniform(-3, 3, (n, 1))).astype(np.float32)
+    qs = (rng.standard_normal((nq, dim)) * 10.0 ** rng.uniform(-2, 2, (nq, 1))).astype(np.float32)
+    bits = rng.random((nq, n)) < rng.uniform(0.05, 0.9, (nq, 1))
+    bits[5] = True     # a query every doc passes
+    bits[6] = False    # and one none does
+    tiles, scales, dim_pad = frk.tile_docs(docs, dtype)
+    assert dim_pad == dim
+    d_t, d_s = dev(tiles), dev(scales) if scales is not None else None
+    store = frk.doc_store(d_t, d_s, dtype, dim_pad, n)
+    d_q = dev(qs)
+    d_m = dev(frk.block_masks_host(bits))
+    d_cand = torch.zeros((nq, n), dtype=torch.int64, device="cuda")
+    d_cnt = torch.zeros(nq, dtype=torch.int32, device="cuda")
+    nb = (n + 31) // 32
+    frk.score_select(store, d_q, nq, d_m, d_m.shape[1], None, 0, 1, nb, d_cand, n, d_cnt)
+    torch.cuda.synchronize()
+    cnt = d_cnt.cpu().numpy()
+    cand = d_cand.cpu().numpy().view(np.uint64)
+    dd, dq = _dequantized(docs, dtype), _dequantized(qs, dtype)
+    for q in range(nq):
+        assert cnt[q] == bits[q].sum(), q
+        sc, ids = frk.decode_keys(cand[q, :cnt[q]])
+        assert np.array_equal(np.sort(ids), np.flatnonzero(bits[q])), q
+        exact = dd[ids] @ dq[q]
+        scale = np.abs(dd[ids] * dq[q]).sum(axis=1)
+        assert np.all(np.abs(sc - exact) <= 2e-5 * scale + 1e-30), (q, np.max(np.abs(sc - exact) / scale))
+
+
 def test_fp32_ties_broken_by_doc_id(frk):
     rng = np.random.default_rng(21)
     n, dim, nq = 3000, 64, 8
@@ -291,7 +344,7 @@ def test_blockmax_probe_and_kth_largest(frk):
     import ctypes
     from hybridsearchsystem_amd import check, lib
     d_qs, d_masks = dev(qs), dev(frk.block_masks_host(bits))  # kept alive until the kernels ran
-    check(lib().frk_score_blockmax(ctypes.byref(store), d_qs.data_ptr(), nq, d_masks.data_ptr(), nq, 0,
+    check(lib().frk_score_blockmax(ctypes.byref(store), d_qs.data_ptr(), nq, d_masks.data_ptr(), d_masks.shape[1], 0,
                                    step, cnt, d_bm.data_ptr(), cnt, None), "blockmax")
     ks = rng.choice([1, 5, 50, 400, 5000], size=nq).astype(np.int32)
     d_kth = torch.zeros(nq, dtype=torch.float32, device="cuda")
