@@ -1,10 +1,10 @@
 """Benchmark: filtered top-100 search over 10M x 256 docs on one MI355X.
 
 One step = one batch of 128 queries through NpuRetrieval::FullRecallSearcher (filter over the posting
-lists, fused MFMA score + mask + floor + compaction, top-k select, D2H). The chosen pipeline (BF16
-docs on MFMA, pruning floor, FP32 rescoring of the top k*m) is timed beside the plain FP32 pipeline on
-the same resident index and query stream; `value` is the chosen pipeline's queries/s, reported only
-if its recall@100 against the plain FP32 results is >= 0.99 (else the plain path's QPS).
+lists, fused MFMA score + mask + floor + compaction, top-k select, D2H). Low-precision pipelines
+(BF16 or FP8 docs on MFMA, a probe or learned pruning floor, FP32 rescoring of the top k*m) are timed
+beside the plain FP32 pipeline on the same resident index and query stream; `value` is the queries/s
+of the fastest one whose recall@100 against the plain FP32 results is >= 0.99 (else the plain path's).
 
 Prints one JSON line. Inputs are synthetic (seeded Gaussian-mixture corpus, Zipf tag densities).
 """
@@ -145,14 +145,32 @@ def calibrate_copy(n_bytes=1 << 30):
     print(json.dumps({"calibration": "copy", "bytes": n_bytes}), flush=True)
 
 
-def learned_leg(table, dens, args, queries, filters, ks, res_p, centroids, seed):
-    """Trains the learned score floor on seeded queries (labels: k'-th scores of the plain FP32
-    method, k' = k * m for m in {1, 4}), times BF16 + learned floor + FP32 rescoring on the bench's
-    query stream, and evaluates prune rate, fallback rate and exactness on held-out queries with
-    k in {10, 100, 1000} and filters down to 0.001% selectivity."""
+DTYPE_NAMES = {0: "f32", 1: "bf16", 2: "fp8"}
+ELEM_BYTES = {0: 4, 1: 2, 2: 1}
+
+
+def timed_leg(table, queries, filters, ks, args, res_p, name, dtype, **kw):
+    """Warms up, then times args.steps batches; returns the run tuple the chooser compares."""
     import torch
+    run_steps(table, queries, filters, ks, args.warmup, args.batch, dtype=dtype, **kw)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    lat, st, res = run_steps(table, queries, filters, ks, args.steps, args.batch, dtype=dtype, **kw)
+    torch.cuda.synchronize()
+    wall = time.perf_counter() - t0
+    rec = recall(res, res_p)
+    qps = args.steps * args.batch / wall
+    log("%s: %.0f q/s, recall %.5f, p99 %.2f ms" % (name, qps, rec, float(np.percentile(lat, 99))))
+    return (name, qps, rec, lat, st, wall, dtype)
+
+
+def learned_leg(table, dens, args, queries, filters, ks, res_p, centroids, seed, legs):
+    """Trains the learned score floor on seeded queries (labels: k'-th scores of the plain FP32
+    method, k' = k * m for m in {1, 4}), times each low-precision leg (dtype, m) with the learned floor
+    and FP32 rescoring on the bench's query stream, and evaluates prune rate, fallback rate and
+    exactness on held-out queries with k in {10, 100, 1000} and filters down to 0.001% selectivity."""
     from hybridsearchsystem_amd import learned, synth
-    from hybridsearchsystem_amd.search import FRK_DTYPE_BF16, FRK_DTYPE_F32
+    from hybridsearchsystem_amd.search import FRK_DTYPE_F32
     S = synth.SEEDS
     t0 = time.time()
     tq = synth.mixture_queries(S["train_queries"], args.learned_train, args.dim, centroids, corpus_seed=seed)
@@ -162,28 +180,27 @@ def learned_leg(table, dens, args, queries, filters, ks, res_p, centroids, seed)
     model, info = learned.train_for_table(table, args.dim, tq, tf, tk, log=log)
     log("learned floor: trained (%s)" % info)
     del tq
-    kw = dict(dtype=FRK_DTYPE_BF16, rescore_factor=4, probe=False, floor_model=model)
-    run_steps(table, queries, filters, ks, args.warmup, args.batch, **kw)
-    torch.cuda.synchronize()
-    t0 = time.perf_counter()
-    lat, st, res = run_steps(table, queries, filters, ks, args.steps, args.batch, **kw)
-    torch.cuda.synchronize()
-    wall = time.perf_counter() - t0
-    rec = recall(res, res_p)
-    qps = args.steps * args.batch / wall
-    timed = {"qps": round(qps, 2), "p50_ms": round(float(np.percentile(lat, 50)), 3),
-             "p99_ms": round(float(np.percentile(lat, 99)), 3), "recall_at_k_vs_plain": round(rec, 5),
-             "candidates_per_query": round(float(np.mean([s["candidates"] for s in st])) / args.batch, 1),
-             "reruns_per_batch": round(float(np.mean([s["reruns"] for s in st])), 3),
-             "stages_ms": {k: round(float(np.mean([s[k] for s in st])), 3) for k in
-                           ("ms_filter", "ms_probe", "ms_score", "ms_select", "ms_rescore", "ms_total")}}
+    runs, timed = [], {}
+    for dname, dtype, m in legs:
+        name = "%s + learned floor + fp32 rescore of top k*%d" % (dname, m)
+        r = timed_leg(table, queries, filters, ks, args, res_p, name, dtype, rescore_factor=m, probe=False,
+                      floor_model=model)
+        runs.append(r)
+        lat, st = r[3], r[4]
+        timed["%s_rescore%d" % (dname, m)] = {
+            "qps": round(r[1], 2), "p50_ms": round(float(np.percentile(lat, 50)), 3),
+            "p99_ms": round(float(np.percentile(lat, 99)), 3), "recall_at_k_vs_plain": round(r[2], 5),
+            "candidates_per_query": round(float(np.mean([s["candidates"] for s in st])) / args.batch, 1),
+            "reruns_per_batch": round(float(np.mean([s["reruns"] for s in st])), 3),
+            "stages_ms": {k: round(float(np.mean([s[k] for s in st])), 3) for k in
+                          ("ms_filter", "ms_probe", "ms_score", "ms_select", "ms_rescore", "ms_total")}}
     # held-out evaluation
     n = args.learned_eval
     eq = synth.mixture_queries(S["eval_queries"], n, args.dim, centroids, corpus_seed=seed)
     ef, _ = synth.random_filters(S["eval_filters"], dens, n, depth=3, sel=(1e-5, 0.6))
     ek = synth.draw_ks(S["eval_kdraw"], n, [10, 100, 1000])
     ev = {}
-    for name, dtype, m in (("plain_fp32", FRK_DTYPE_F32, 1), ("bf16_rescore4", FRK_DTYPE_BF16, 4)):
+    for name, dtype, m in [("plain_fp32", FRK_DTYPE_F32, 1)] + [("%s_rescore%d" % (d, m), t, m) for d, t, m in legs]:
         cand = {"learned": 0}
         passing = reruns = exact = hits = tot = 0
         for lo in range(0, n, args.batch):
@@ -209,8 +226,7 @@ def learned_leg(table, dens, args, queries, filters, ks, res_p, centroids, seed)
         log("learned floor eval %s: %s" % (name, ev[name]))
     out = {"model": "MLP %d+32 -> 256 -> 256 -> 1, BF16 MFMA training and inference" % args.dim, "train": info,
            "timed_k100": timed, "heldout_k_10_100_1000": ev}
-    run = ("bf16 + learned floor + fp32 rescore of top k*4", qps, rec, lat, st, wall)
-    return out, run
+    return out, runs
 
 
 def main():
@@ -231,8 +247,9 @@ def main():
     ap.add_argument("--no-learned", action="store_true", help="skip training and timing the learned floor")
     ap.add_argument("--learned-train", type=int, default=200_000, help="seeded training queries")
     ap.add_argument("--learned-eval", type=int, default=8192, help="held-out queries (k in {10,100,1000})")
-    ap.add_argument("--traffic-profile", default=os.path.join(ROOT, "profiles", "score_bf16_10Mx256_q128_traffic.md"),
-                    help="rocprofv3 counter summary holding the main score kernel's HBM bytes per launch")
+    ap.add_argument("--traffic-profile", default=None,
+                    help="rocprofv3 counter summary holding the main score kernel's HBM bytes per launch "
+                         "(default: profiles/score_<dtype>_10Mx256_q128_traffic.md of the chosen pipeline)")
     args = ap.parse_args()
     if args.calibrate_copy:
         calibrate_copy()
@@ -242,70 +259,66 @@ def main():
         sys.exit(2)
     import torch
     from hybridsearchsystem_amd import synth
-    from hybridsearchsystem_amd.search import FRK_DTYPE_BF16, FRK_DTYPE_F32
+    from hybridsearchsystem_amd.search import FRK_DTYPE_BF16, FRK_DTYPE_F32, FRK_DTYPE_FP8E4M3
 
     n_tags, centroids, seed = 1024, 1024, synth.SEEDS["corpus"]
     cpu_docs = min(args.cpu_docs or args.docs, args.docs)
-    table, dens, off, ids, head = build_table(args.docs, args.dim, n_tags, centroids, (FRK_DTYPE_BF16,), seed,
-                                              0 if args.no_cpu else cpu_docs)
+    table, dens, off, ids, head = build_table(args.docs, args.dim, n_tags, centroids,
+                                              (FRK_DTYPE_BF16, FRK_DTYPE_FP8E4M3), seed, 0 if args.no_cpu else cpu_docs)
     queries = synth.mixture_queries(synth.SEEDS["queries"], args.queries, args.dim, centroids, corpus_seed=seed)
     filters, sels = synth.random_filters(synth.SEEDS["filters"], dens, args.queries, depth=3, sel=(0.0001, 0.6))
     ks = np.full(args.queries, args.k, np.uint32)
 
     plain_kw = dict(dtype=FRK_DTYPE_F32)
-    chosen_kw = dict(dtype=FRK_DTYPE_BF16, rescore_factor=4)
     run_steps(table, queries, filters, ks, args.warmup, args.batch, **plain_kw)
     torch.cuda.synchronize()
     t0 = time.perf_counter()
     lat_p, st_p, res_p = run_steps(table, queries, filters, ks, args.steps, args.batch, **plain_kw)
     torch.cuda.synchronize()
     wall_p = time.perf_counter() - t0
-    run_steps(table, queries, filters, ks, args.warmup, args.batch, **chosen_kw)
-    torch.cuda.synchronize()
-    t0 = time.perf_counter()
-    lat_c, st_c, res_c = run_steps(table, queries, filters, ks, args.steps, args.batch, **chosen_kw)
-    torch.cuda.synchronize()
-    wall_c = time.perf_counter() - t0
-    rec = recall(res_c, res_p)
     qps_p = args.steps * args.batch / wall_p
-    qps_c = args.steps * args.batch / wall_c
-    learned_out, lrun = None, None
+    # low-precision legs: (store, rescoring factor m); FP32 rescoring of the top k*m recovers recall
+    legs = [("bf16", FRK_DTYPE_BF16, 4), ("fp8", FRK_DTYPE_FP8E4M3, 4), ("fp8", FRK_DTYPE_FP8E4M3, 8)]
+    runs = [timed_leg(table, queries, filters, ks, args, res_p, "%s + probe floor + fp32 rescore of top k*%d" % (d, m),
+                      t, rescore_factor=m) for d, t, m in legs]
+    learned_out = None
     if not args.no_learned:
-        learned_out, lrun = learned_leg(table, dens, args, queries, filters, ks, res_p, centroids, seed)
+        learned_out, lruns = learned_leg(table, dens, args, queries, filters, ks, res_p, centroids, seed, legs)
+        runs += lruns
     # the fastest low-precision pipeline whose recall@k against plain FP32 is >= 0.99
-    runs = [("bf16 + probe floor + fp32 rescore of top k*4", qps_c, rec, lat_c, st_c, wall_c)]
-    if lrun is not None:
-        runs.append(lrun)
     ok_runs = [r for r in runs if r[2] >= 0.99]
     chosen_ok = bool(ok_runs)
     best = max(ok_runs, key=lambda r: r[1]) if ok_runs else None
     value = best[1] if chosen_ok else qps_p
     lat = best[3] if chosen_ok else lat_p
     st = best[4] if chosen_ok else st_p
-    wall_c = best[5] if chosen_ok else wall_c
+    wall_c = best[5] if chosen_ok else wall_p
+    best_dtype = best[6] if chosen_ok else FRK_DTYPE_F32
 
     # roofline of the dominant kernel: the main fused score pass, timed by HIP events recorded on the
     # searcher's stream around its launch. Algorithmic bytes per launch: the doc stream once, the
-    # block-major filter masks (4 bytes per (32-doc block, query row of the padded batch)), and the
-    # 8-byte candidate keys written.
+    # per-doc filter words (4 bytes per (32-doc block, query row of the padded batch)), FP8's per-doc
+    # scales (4 bytes per doc), and the 8-byte candidate keys written.
     n, d, q = args.docs, args.dim, args.batch
     n_blocks = (n + 31) // 32
-    doc_bytes = n_blocks * 32 * d * (2 if chosen_ok else 4)
+    doc_bytes = n_blocks * 32 * d * ELEM_BYTES[best_dtype] + (n_blocks * 32 * 4 if best_dtype == FRK_DTYPE_FP8E4M3 else 0)
     mask_bytes = n_blocks * ((q + 31) // 32 * 32) * 4
     cand_bytes = float(np.mean([s["candidates"] for s in st])) * 8
     alg_bytes = doc_bytes + mask_bytes + cand_bytes
     ms_score = float(np.mean([s["ms_score"] for s in st]))
     achieved = alg_bytes / (ms_score * 1e-3) / 1e9
     roof = dict(bound="hbm", achieved=round(achieved, 1), peak=HBM_PEAK_GBS, unit="GB/s",
-                frac=round(achieved / HBM_PEAK_GBS, 4), traffic=traffic_from_profile(args.traffic_profile),
+                frac=round(achieved / HBM_PEAK_GBS, 4),
+                traffic=traffic_from_profile(args.traffic_profile or os.path.join(
+                    ROOT, "profiles", "score_%s_10Mx256_q128_traffic.md" % DTYPE_NAMES[best_dtype])),
                 alg_bytes=int(alg_bytes), kernel_ms=round(ms_score, 4))
     ms_score_p = float(np.mean([s["ms_score"] for s in st_p]))
     flops_p = 2.0 * q * n * d
     tf_p = flops_p / (ms_score_p * 1e-3) / 1e12
     out = {
         "metric": METRIC, "value": round(value, 2), "unit": "queries/s", "n_gpus": 1, "steps": args.steps,
-        "warmup": args.warmup, "ms_per_step": round(wall_c / args.steps * 1e3 if chosen_ok else wall_p / args.steps * 1e3, 3),
-        "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "bf16" if chosen_ok else "f32",
+        "warmup": args.warmup, "ms_per_step": round(wall_c / args.steps * 1e3, 3),
+        "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": DTYPE_NAMES[best_dtype],
         "data": "synthetic (seeded Gaussian mixture, Zipf tags)",
         "config": {"workload": "10M x 256 filtered top-%d, batch %d" % (args.k, args.batch), "docs": n, "dim": d,
                    "batch": q, "k": args.k, "tags": n_tags, "selectivity": [0.0001, 0.6]},

@@ -237,11 +237,13 @@ bool FullRecallSearcher::RunDevice(const VectorFieldData& vf, std::vector<AggrDe
     const uint32_t m = lowp ? std::max(1u, m_options.rescoreFactor) : 1;
     uint32_t kmax = 1;
     for (uint32_t k : m_topKs) kmax = std::max(kmax, k);
-    if (kmax * m > FRK_SELECT_MAX_K) {
-        FRK_LOG_ERROR("topK " << kmax << " x rescore factor " << m << " exceeds " << FRK_SELECT_MAX_K);
+    if (kmax > FRK_SELECT_MAX_K) {
+        FRK_LOG_ERROR("topK " << kmax << " exceeds " << FRK_SELECT_MAX_K);
         return false;
     }
-    const uint32_t kSel = kmax * m;  // keys selected on the main pass
+    // keys a query selects on the main pass: k * m, at most what one select takes (never below k)
+    auto kPrime = [&](uint32_t k) { return std::min<uint32_t>(k * m, std::max<uint32_t>(k, FRK_SELECT_MAX_K)); };
+    const uint32_t kSel = kPrime(kmax);
 
     // ---- compile the filters
     std::vector<frk_filter_query> descs(Q);
@@ -323,7 +325,7 @@ bool FullRecallSearcher::RunDevice(const VectorFieldData& vf, std::vector<AggrDe
     uint32_t* hkm = reinterpret_cast<uint32_t*>(hb + oKm);
     for (uint32_t q = 0; q < Q; q++) {
         hk[q] = m_topKs[q];
-        hkm[q] = m_topKs[q] * m;
+        hkm[q] = kPrime(m_topKs[q]);
     }
     const bool callerFloor = m_options.floors != nullptr;
     if (callerFloor) std::memcpy(hb + oFloor, m_options.floors, sizeof(float) * Q);
@@ -334,13 +336,13 @@ bool FullRecallSearcher::RunDevice(const VectorFieldData& vf, std::vector<AggrDe
         for (uint32_t q = 0; q < Q; q++) {
             const float* qv = &m_queries[static_cast<size_t>(q) * m_dim];
             QueryNode* tree = (descs[q].flags & FRK_FILTER_MATCH_ALL) ? nullptr : m_trees[q];
-            FloorFeatures(*m_table, tree, m_topKs[q] * m, qv, m_dim, f);
+            FloorFeatures(*m_table, tree, kPrime(m_topKs[q]), qv, m_dim, f);
             float* row = hx + static_cast<size_t>(q) * inDim;
             std::memcpy(row, qv, sizeof(float) * m_dim);
             for (uint32_t j = 0; j < kFloorFeatures; j++) row[m_dim + j] = (f[j] - model->feat_mean[j]) / model->feat_std[j];
             // too few docs expected to pass for a floor to pay off (f[2] = log10 expected passing)
             const bool allow = std::pow(10.0, static_cast<double>(f[2])) >=
-                               static_cast<double>(model->min_passing_ratio) * m_topKs[q] * m;
+                               static_cast<double>(model->min_passing_ratio) * kPrime(m_topKs[q]);
             // margin of this k's decade (f[1] = log10 k) when the model has per-decade margins
             float margin = model->margin;
             if (model->margin_by_k) {
@@ -361,7 +363,7 @@ bool FullRecallSearcher::RunDevice(const VectorFieldData& vf, std::vector<AggrDe
     if (autoStep) probeStep = static_cast<uint32_t>(std::min<uint64_t>(64, std::max<uint64_t>(1, nBlocks / (64ull * kSel))));
     // a query whose k*m exceeds the sampled blocks gets a -inf floor (no pruning), which stays exact
     uint32_t kSelMin = kSel;
-    for (uint32_t k : m_topKs) kSelMin = std::min(kSelMin, k * m);
+    for (uint32_t k : m_topKs) kSelMin = std::min(kSelMin, kPrime(k));
     const bool useProbe = m_options.probe && !floorChecked && probeStep > 1 && nBlocks / probeStep >= kSelMin;
     uint32_t cap = m_options.candCap;
     if (cap == 0) {
@@ -553,7 +555,7 @@ bool FullRecallSearcher::RunDevice(const VectorFieldData& vf, std::vector<AggrDe
     for (uint32_t q = 0; q < Q; q++) {
         m_passing[q] = hCnt[cFilter + q];
         m_stats.candidates += hCnt[cCand + q];
-        const uint32_t want = std::min(m_topKs[q] * m, m_passing[q]);
+        const uint32_t want = std::min(kPrime(m_topKs[q]), m_passing[q]);
         const bool overflow = hCnt[cCand + q] > cap;
         const bool shortFloor = floorChecked && std::min(hCnt[cCand + q], cap) < want;
         if (overflow) m_stats.overflowed++;

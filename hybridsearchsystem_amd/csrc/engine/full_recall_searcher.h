@@ -48,7 +48,7 @@ struct AggrDeviceResult {
 // Scoring precision and pruning for a batch.
 struct SearchOptions {
     uint32_t dtype = FRK_DTYPE_F32;  // store scored on the main pass
-    uint32_t rescoreFactor = 4;      // low precision: FP32 rescoring of the top k * m (m = this)
+    uint32_t rescoreFactor = 4;      // low precision: FP32 rescoring of the top min(k * m, 4096) (m = this)
     bool probe = true;               // sampled k-th best score as the pruning floor (exact)
     uint32_t probeStep = 0;          // sample every probeStep-th doc block; 0 = chosen from k and N
     const float* floors = nullptr;   // optional caller floor per query (e.g. a learned bound); the

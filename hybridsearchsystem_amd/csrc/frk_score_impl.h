@@ -159,7 +159,11 @@ __device__ __forceinline__ void StageQueries(const ScoreArgs& A, u32x4* a_lds, f
 // epilogue appears once per body position that ends a block. Loads are unconditional (a step past
 // the end reloads the last step's addresses), so the code between a load and its use is straight-line
 // and each wait counts the loads still allowed in flight exactly.
-constexpr int kRing = 4;
+#ifndef FRK_SCORE_RING
+#define FRK_SCORE_RING 4
+#endif
+constexpr int kRing = FRK_SCORE_RING;
+static_assert(kRing >= 2, "the ring needs a slot in use and one in flight");
 
 // f(integral_constant<int, I>) for each I of the sequence, in order.
 template <typename F, int... I>
@@ -508,12 +512,11 @@ __global__ __launch_bounds__(kThreads, FRK_SCORE_MIN_BLOCKS) void ScoreKernel(Sc
         if constexpr (GL == 0) LoadMasks<DT, RB>(A, q0, blk, mring[SL], sring[SL]);
     };
     // the first kRing - 1 steps, in step order (each wait below counts on that order)
-    load(std::integral_constant<int, 0>{}, std::integral_constant<int, 0 % NG>{}, 0);
-    __builtin_amdgcn_sched_barrier(0);
-    load(std::integral_constant<int, 1>{}, std::integral_constant<int, 1 % NG>{}, 1);
-    __builtin_amdgcn_sched_barrier(0);
-    load(std::integral_constant<int, 2>{}, std::integral_constant<int, 2 % NG>{}, 2);
-    __builtin_amdgcn_sched_barrier(0);
+    StaticFor([&](auto s_c) __attribute__((always_inline)) {
+        constexpr int st = decltype(s_c)::value;
+        load(std::integral_constant<int, st>{}, std::integral_constant<int, st % NG>{}, st);
+        __builtin_amdgcn_sched_barrier(0);
+    }, std::make_integer_sequence<int, kRing - 1>{});
     floatx16 acc[RB];  // set from the filter words at the start of every block
     // step s of the loop body (body starts at a block boundary `base`)
     auto step_fn = [&](auto s_c, uint64_t base) __attribute__((always_inline)) {
@@ -589,13 +592,21 @@ int Launch(const ScoreArgs& A, hipStream_t stream, uint32_t grid_x) {
     return FRK_OK;
 }
 
+#ifndef FRK_SCORE_MAX_RB
+#define FRK_SCORE_MAX_RB 4  // most 32-query row blocks one workgroup takes (1, 2 or 4)
+#endif
+
 // Instantiated shapes: CH = 16-byte chunks per doc (FP32 dims 32..1024, BF16 64..2048, FP8 128..4096
 // at the listed counts; kScoreShapes lists them for frk_store_dim, which pads a store's dims up to one). RB = 32-query row blocks per workgroup, at most what keeps the query tile
 // (RB * CH KiB) within 128 KiB of the 160 KiB LDS.
 template <int DT, bool kBlockMax>
 int Dispatch(const ScoreArgs& A, uint32_t ch, hipStream_t stream, uint32_t grid_x) {
     const uint32_t q = A.n_queries;
-    const int rb = q > 64 ? 4 : (q > 32 ? 2 : 1);
+    int rb = q > 64 ? 4 : (q > 32 ? 2 : 1);
+    if (rb > FRK_SCORE_MAX_RB) rb = FRK_SCORE_MAX_RB;
+    // FP8 at four row blocks holds too many registers for two waves per SIMD (measured slower than two
+    // query groups of two row blocks sharing an XCD's L2)
+    if (DT == FRK_DTYPE_FP8E4M3 && rb > 2) rb = 2;
 #define FRK_CASE(CHV)                                                                           \
     if (ch == CHV) {                                                                            \
         if constexpr (CHV * 4 <= 128) {                                                         \

@@ -328,7 +328,7 @@ int frk_compile_filter(const frk_table* t, const char* field, const char* expr, 
 
 typedef struct {
     uint32_t dtype;           /* scoring store: FRK_DTYPE_F32 (plain), _BF16 or _FP8E4M3 */
-    uint32_t rescore_factor;  /* low precision: FP32 rescoring of the top k * m */
+    uint32_t rescore_factor;  /* low precision: FP32 rescoring of the top min(k * m, FRK_SELECT_MAX_K) */
     int32_t probe;            /* sampled k-th-best floor (exact pruning) */
     uint32_t probe_step;      /* 0 = automatic */
     const float* floors;      /* host, per query, or NULL: caller score floors */
