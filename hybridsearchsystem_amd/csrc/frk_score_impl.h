@@ -536,6 +536,11 @@ __global__ __launch_bounds__(kThreads, FRK_SCORE_MIN_BLOCKS) void ScoreKernel(Sc
         }
         ComputeGroup<DT, CH, RB>(a_lds, g, ring[S], acc, qexp, dexp);
         if constexpr (g == NG - 1) {
+            // every row block's MFMAs are issued before the first epilogue; left alone, the compiler
+            // sinks a row block's MFMAs past the epilogue branches of the row blocks before it, and the
+            // MFMA and epilogue work of one wave then no longer overlap
+#pragma unroll
+            for (int rb = 0; rb < RB; rb++) asm volatile("" : "+v"(acc[rb]));
             const uint64_t i = block_index(step);
             const uint64_t blk = A.block_begin + i * A.block_step;
             if constexpr (kBlockMax)
@@ -595,6 +600,9 @@ int Launch(const ScoreArgs& A, hipStream_t stream, uint32_t grid_x) {
 #ifndef FRK_SCORE_MAX_RB
 #define FRK_SCORE_MAX_RB 4  // most 32-query row blocks one workgroup takes (1, 2 or 4)
 #endif
+#ifndef FRK_SCORE_FP8_PROBE_MAX_RB
+#define FRK_SCORE_FP8_PROBE_MAX_RB 2
+#endif
 
 // Instantiated shapes: CH = 16-byte chunks per doc (FP32 dims 32..1024, BF16 64..2048, FP8 128..4096
 // at the listed counts; kScoreShapes lists them for frk_store_dim, which pads a store's dims up to one). RB = 32-query row blocks per workgroup, at most what keeps the query tile
@@ -604,9 +612,8 @@ int Dispatch(const ScoreArgs& A, uint32_t ch, hipStream_t stream, uint32_t grid_
     const uint32_t q = A.n_queries;
     int rb = q > 64 ? 4 : (q > 32 ? 2 : 1);
     if (rb > FRK_SCORE_MAX_RB) rb = FRK_SCORE_MAX_RB;
-    // FP8 at four row blocks holds too many registers for two waves per SIMD (measured slower than two
-    // query groups of two row blocks sharing an XCD's L2)
-    if (DT == FRK_DTYPE_FP8E4M3 && rb > 2) rb = 2;
+    // FP8 probe pass: two query groups of two row blocks (sharing an XCD's L2) measured faster
+    if (DT == FRK_DTYPE_FP8E4M3 && kBlockMax && rb > FRK_SCORE_FP8_PROBE_MAX_RB) rb = FRK_SCORE_FP8_PROBE_MAX_RB;
 #define FRK_CASE(CHV)                                                                           \
     if (ch == CHV) {                                                                            \
         if constexpr (CHV * 4 <= 128) {                                                         \
