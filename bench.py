@@ -5,6 +5,8 @@ lists, fused MFMA score + mask + floor + compaction, top-k select, D2H). Low-pre
 (BF16 or FP8 docs on MFMA, a probe or learned pruning floor, FP32 rescoring of the top k*m) are timed
 beside the plain FP32 pipeline on the same resident index and query stream; `value` is the queries/s
 of the fastest one whose recall@100 against the plain FP32 results is >= 0.99 (else the plain path's).
+Each batch's device sequence is replayed from a hipGraph captured per batch shape (--eager: launched
+kernel by kernel); per-stage kernel times come from an eager pass over the same stream.
 
 Prints one JSON line. Inputs are synthetic (seeded Gaussian-mixture corpus, Zipf tag densities).
 """
@@ -150,8 +152,10 @@ ELEM_BYTES = {0: 4, 1: 2, 2: 1}
 
 
 def timed_leg(table, queries, filters, ks, args, res_p, name, dtype, **kw):
-    """Warms up, then times args.steps batches; returns the run tuple the chooser compares."""
+    """Warms up, then times args.steps batches (device sequence replayed from a hipGraph per batch
+    shape unless --eager); returns the run tuple the chooser compares."""
     import torch
+    kw["graph"] = not args.eager
     run_steps(table, queries, filters, ks, args.warmup, args.batch, dtype=dtype, **kw)
     torch.cuda.synchronize()
     t0 = time.perf_counter()
@@ -161,7 +165,7 @@ def timed_leg(table, queries, filters, ks, args, res_p, name, dtype, **kw):
     rec = recall(res, res_p)
     qps = args.steps * args.batch / wall
     log("%s: %.0f q/s, recall %.5f, p99 %.2f ms" % (name, qps, rec, float(np.percentile(lat, 99))))
-    return (name, qps, rec, lat, st, wall, dtype)
+    return (name, qps, rec, lat, st, wall, dtype, dict(kw, dtype=dtype))
 
 
 def learned_leg(table, dens, args, queries, filters, ks, res_p, centroids, seed, legs):
@@ -192,8 +196,7 @@ def learned_leg(table, dens, args, queries, filters, ks, res_p, centroids, seed,
             "p99_ms": round(float(np.percentile(lat, 99)), 3), "recall_at_k_vs_plain": round(r[2], 5),
             "candidates_per_query": round(float(np.mean([s["candidates"] for s in st])) / args.batch, 1),
             "reruns_per_batch": round(float(np.mean([s["reruns"] for s in st])), 3),
-            "stages_ms": {k: round(float(np.mean([s[k] for s in st])), 3) for k in
-                          ("ms_filter", "ms_probe", "ms_score", "ms_select", "ms_rescore", "ms_total")}}
+            "device_ms_per_batch": round(float(np.mean([s["ms_total"] for s in st])), 3)}
     # held-out evaluation
     n = args.learned_eval
     eq = synth.mixture_queries(S["eval_queries"], n, args.dim, centroids, corpus_seed=seed)
@@ -245,6 +248,7 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--calibrate-copy", action="store_true", help="run one 1 GiB copy and exit")
     ap.add_argument("--no-learned", action="store_true", help="skip training and timing the learned floor")
+    ap.add_argument("--eager", action="store_true", help="launch every batch's kernels one by one (no hipGraph replay)")
     ap.add_argument("--learned-train", type=int, default=200_000, help="seeded training queries")
     ap.add_argument("--learned-eval", type=int, default=8192, help="held-out queries (k in {10,100,1000})")
     ap.add_argument("--traffic-profile", default=None,
@@ -269,7 +273,7 @@ def main():
     filters, sels = synth.random_filters(synth.SEEDS["filters"], dens, args.queries, depth=3, sel=(0.0001, 0.6))
     ks = np.full(args.queries, args.k, np.uint32)
 
-    plain_kw = dict(dtype=FRK_DTYPE_F32)
+    plain_kw = dict(dtype=FRK_DTYPE_F32, graph=not args.eager)
     run_steps(table, queries, filters, ks, args.warmup, args.batch, **plain_kw)
     torch.cuda.synchronize()
     t0 = time.perf_counter()
@@ -277,6 +281,8 @@ def main():
     torch.cuda.synchronize()
     wall_p = time.perf_counter() - t0
     qps_p = args.steps * args.batch / wall_p
+    # per-stage HIP-event times of the plain pipeline: an eager pass over the same stream
+    _, st_pe, _ = run_steps(table, queries, filters, ks, args.steps, args.batch, dtype=FRK_DTYPE_F32, graph=False)
     # low-precision legs: (store, rescoring factor m); FP32 rescoring of the top k*m recovers recall
     legs = [("bf16", FRK_DTYPE_BF16, 4), ("fp8", FRK_DTYPE_FP8E4M3, 4), ("fp8", FRK_DTYPE_FP8E4M3, 8)]
     runs = [timed_leg(table, queries, filters, ks, args, res_p, "%s + probe floor + fp32 rescore of top k*%d" % (d, m),
@@ -294,6 +300,12 @@ def main():
     st = best[4] if chosen_ok else st_p
     wall_c = best[5] if chosen_ok else wall_p
     best_dtype = best[6] if chosen_ok else FRK_DTYPE_F32
+    # per-stage HIP-event times (a replayed graph carries no events between its kernels): the chosen
+    # pipeline once more, eagerly, over the same query stream; its kernels are the ones the graph replays
+    if chosen_ok:
+        _, st_e, _ = run_steps(table, queries, filters, ks, args.steps, args.batch, **dict(best[7], graph=False))
+    else:
+        st_e = st_pe
 
     # roofline of the dominant kernel: the main fused score pass, timed by HIP events recorded on the
     # searcher's stream around its launch. Algorithmic bytes per launch: the doc stream once, the
@@ -303,16 +315,16 @@ def main():
     n_blocks = (n + 31) // 32
     doc_bytes = n_blocks * 32 * d * ELEM_BYTES[best_dtype] + (n_blocks * 32 * 4 if best_dtype == FRK_DTYPE_FP8E4M3 else 0)
     mask_bytes = n_blocks * ((q + 31) // 32 * 32) * 4
-    cand_bytes = float(np.mean([s["candidates"] for s in st])) * 8
+    cand_bytes = float(np.mean([s["candidates"] for s in st_e])) * 8
     alg_bytes = doc_bytes + mask_bytes + cand_bytes
-    ms_score = float(np.mean([s["ms_score"] for s in st]))
+    ms_score = float(np.mean([s["ms_score"] for s in st_e]))
     achieved = alg_bytes / (ms_score * 1e-3) / 1e9
     roof = dict(bound="hbm", achieved=round(achieved, 1), peak=HBM_PEAK_GBS, unit="GB/s",
                 frac=round(achieved / HBM_PEAK_GBS, 4),
                 traffic=traffic_from_profile(args.traffic_profile or os.path.join(
                     ROOT, "profiles", "score_%s_10Mx256_q128_traffic.md" % DTYPE_NAMES[best_dtype])),
                 alg_bytes=int(alg_bytes), kernel_ms=round(ms_score, 4))
-    ms_score_p = float(np.mean([s["ms_score"] for s in st_p]))
+    ms_score_p = float(np.mean([s["ms_score"] for s in st_pe]))
     flops_p = 2.0 * q * n * d
     tf_p = flops_p / (ms_score_p * 1e-3) / 1e12
     out = {
@@ -332,8 +344,12 @@ def main():
                    "speedup_vs_plain": round(value / qps_p, 3),
                    "candidates": {r[0]: {"qps": round(r[1], 2), "recall": round(r[2], 5),
                                          "p99_ms": round(float(np.percentile(r[3], 99)), 3)} for r in runs}},
-        "stages_ms": {k: round(float(np.mean([s[k] for s in st])), 3) for k in
+        "graph_replay": not args.eager,
+        "device_ms_per_batch": round(float(np.mean([s["ms_total"] for s in st])), 3),
+        "stages_ms": {k: round(float(np.mean([s[k] for s in st_e])), 3) for k in
                       ("ms_filter", "ms_probe", "ms_score", "ms_select", "ms_rescore", "ms_total")},
+        "stages_note": "stages_ms and roofline.kernel_ms: HIP events on the searcher stream in an eager pass of the "
+                       "chosen pipeline over the same query stream (a replayed graph has no events between kernels)",
         "candidates_per_query": round(float(np.mean([s["candidates"] for s in st])) / q, 1),
         "plain_candidates_per_query": round(float(np.mean([s["candidates"] for s in st_p])) / q, 1),
         "probe_step": int(st[-1]["probe_step"]), "plain_probe_step": int(st_p[-1]["probe_step"]),
