@@ -9,7 +9,8 @@
 //
 // Work split: one workgroup of 512 threads per (query, segment, chunk of 512*W words); each thread
 // owns W words (64 docs each), at w0 + t + 512 j. The kernel runs in three phases:
-//   1. the addresses of the first S = 16 / W operands go to LDS;
+//   1. the addresses of the first S operands go to LDS (S slots, at most 16 / W: as many as the
+//      batch's largest program needs);
 //   2. each of those operands is staged whole into an LDS slot holding the chunk's words:
 //      DENSE_BITMAP (0) words are copied with coalesced 8-byte loads, DENSE_BYTEMAP (2) bytes are
 //      packed to bits, SPARSE_BITMAP (1) and SPARSE_PACKED (4) units are scattered with LDS atomic
@@ -19,7 +20,7 @@
 //   3. the program runs per word with its stack in registers and its operands read from LDS.
 // Operands past the S slots (programs with more postings than the launch was sized for) are read
 // per word straight from HBM: correct, slower.
-// LDS: S slots x 512 W words x 8 B = 64 KiB for every W, so two workgroups (16 waves) fit a CU.
+// LDS: S slots x 512 W words x 8 B, at most 64 KiB.
 #include <stdarg.h>
 #include <stdio.h>
 
@@ -29,7 +30,8 @@ namespace frk {
 namespace {
 
 constexpr int kThreads = 512;
-constexpr int kSlotWords = 8192;  // S * C: staged operand words per workgroup
+constexpr int kMaxSlots = 16;  // S * 512 W <= 8192 words (64 KiB)
+constexpr uint32_t kLayoutEmpty = 15;  // a staged operand with no hits (absent posting or hitCount 0)
 constexpr uint32_t kStatusBadProgram = 1u;
 constexpr uint32_t kStatusTooManyPostings = 2u;
 
@@ -179,11 +181,11 @@ __global__ __launch_bounds__(kThreads) void FilterKernel(const frk_filter_query*
                                                          const uint64_t* __restrict__ table, uint32_t seg_docs,
                                                          uint64_t n_docs, uint64_t* __restrict__ out,
                                                          uint64_t out_stride, uint32_t* __restrict__ counts,
-                                                         uint32_t* __restrict__ status) {
-    constexpr uint32_t C = kThreads * W;    // words per chunk
-    constexpr uint32_t S = kSlotWords / C;  // staged operands
-    __shared__ uint64_t slots[S][C];
-    __shared__ uint64_t head_addr[S];
+                                                         uint32_t* __restrict__ status, uint32_t S) {
+    constexpr uint32_t C = kThreads * W;  // words per chunk
+    extern __shared__ uint64_t slot_words[];  // S slots of C words
+    __shared__ uint64_t head_addr[kMaxSlots];
+    __shared__ uint32_t head_layout[kMaxSlots], head_len[kMaxSlots], head_hits[kMaxSlots];
     __shared__ uint32_t wave_pop[kThreads / 64];
 
     const uint32_t q = blockIdx.z;
@@ -213,38 +215,69 @@ __global__ __launch_bounds__(kThreads) void FilterKernel(const frk_filter_query*
     } else {
         const uint64_t* tab = table + Q.table_off + static_cast<uint64_t>(seg) * Q.n_postings;
         const uint32_t n_staged = Q.n_postings < S ? Q.n_postings : S;
-        // ---- phase 1: operand addresses to LDS
-        if (t < n_staged) head_addr[t] = tab[t];
+        // ---- phase 1: the staged operands' heads to LDS, read in parallel
+        if (t < n_staged) {
+            head_addr[t] = tab[t];
+            const PostingHead h = ReadHead(tab[t]);
+            head_layout[t] = h.hits == 0 ? kLayoutEmpty : h.layout;
+            head_len[t] = h.len;
+            head_hits[t] = h.hits;
+        }
         __syncthreads();
 
-        // ---- phase 2: stage every slotted operand's chunk words
+        // ---- phase 2: stage every slotted operand's chunk words in rounds over all operands, each
+        // round's loads independent of the others', so their latencies overlap:
+        //   a. dense operands' W words per thread go through registers into their slots; the slots of
+        //      sparse and empty operands are cleared;
+        //   b. (barrier) sparse operands scatter into their slots;
+        //   c. (barrier) the program reads the slots.
         const bool whole = C >= seg_words;  // the chunk covers the segment: sparse lists need no search
-        for (uint32_t p = 0; p < n_staged; p++) {
-            const PostingHead h = ReadHead(head_addr[p]);
-            uint64_t* slot = slots[p];
-            if (h.hits == 0 || h.layout == 0 || h.layout == 2) {
-                uint64_t v[W];
+        {
+            constexpr int kP = kMaxSlots / W;  // most slots at this W
+            uint64_t v[kP][W];
+#pragma unroll
+            for (int p = 0; p < kP; p++) {
+#pragma unroll
+                for (int j = 0; j < W; j++) v[p][j] = 0;
+                if (static_cast<uint32_t>(p) >= n_staged) continue;
+                const uint32_t lay = head_layout[p];
+                if (lay != 0 && lay != 2) continue;
+                PostingHead h;
+                h.payload = reinterpret_cast<const uint8_t*>(head_addr[p]) + 8;
+                h.layout = lay;
+                h.hits = head_hits[p];
+                h.len = head_len[p];
 #pragma unroll
                 for (int j = 0; j < W; j++) {
                     const uint32_t w = w0 + t + kThreads * j;
-                    v[j] = 0;
-                    if (h.hits != 0 && w < seg_words) {
-                        if (h.layout == 0) {
-                            if (8ull * w < h.len) v[j] = reinterpret_cast<const uint64_t*>(h.payload)[w];
+                    if (w < seg_words) {
+                        if (lay == 0) {
+                            if (8ull * w < h.len) v[p][j] = reinterpret_cast<const uint64_t*>(h.payload)[w];
                         } else {
-                            v[j] = BytemapWord(h, w);
+                            v[p][j] = BytemapWord(h, w);
                         }
                     }
                 }
-#pragma unroll
-                for (int j = 0; j < W; j++) slot[t + kThreads * j] = v[j];
-                continue;
             }
-            // sparse layouts: clear, then scatter
 #pragma unroll
-            for (int j = 0; j < W; j++) slot[t + kThreads * j] = 0;
-            __syncthreads();
-            if (h.layout == 1 || h.layout == 4) {
+            for (int p = 0; p < kP; p++) {
+                if (static_cast<uint32_t>(p) >= n_staged) continue;
+                uint64_t* slot = slot_words + static_cast<size_t>(p) * C;
+#pragma unroll
+                for (int j = 0; j < W; j++) slot[t + kThreads * j] = v[p][j];
+            }
+        }
+        __syncthreads();
+        for (uint32_t p = 0; p < n_staged; p++) {
+            const uint32_t lay = head_layout[p];
+            if (lay != 1 && lay != 3 && lay != 4) continue;
+            PostingHead h;
+            h.payload = reinterpret_cast<const uint8_t*>(head_addr[p]) + 8;
+            h.layout = lay;
+            h.hits = head_hits[p];
+            h.len = head_len[p];
+            uint64_t* slot = slot_words + static_cast<size_t>(p) * C;
+            if (lay == 1 || lay == 4) {
                 const uint32_t k = UnitCount(h);
                 uint32_t lo = 0, hi = k;
                 if (!whole) CoopLowerBounds(h, k, 4 * w0, 4 * (w0 + C), lo, hi);
@@ -256,7 +289,7 @@ __global__ __launch_bounds__(kThreads) void FilterKernel(const frk_filter_query*
                         atomicOr(reinterpret_cast<unsigned long long*>(&slot[wd - w0]),
                                  static_cast<unsigned long long>(mask) << (16 * (unit & 3)));
                 }
-            } else if (h.layout == 3) {
+            } else {
                 const uint32_t* ids = reinterpret_cast<const uint32_t*>(h.payload);
                 const uint32_t n = h.len / 4;
                 for (uint32_t i = t; i < n; i += kThreads) {
@@ -278,7 +311,7 @@ __global__ __launch_bounds__(kThreads) void FilterKernel(const frk_filter_query*
             const uint32_t wl = t + kThreads * j;
             const uint32_t w = w0 + wl;
             auto operand = [&](uint32_t p) -> uint64_t {
-                if (p < S) return slots[p][wl];
+                if (p < S) return slot_words[static_cast<size_t>(p) * C + wl];
                 return w < seg_words ? OperandWordDirect(ReadHead(tab[p]), w) : 0;
             };
             uint64_t stack[FRK_FILTER_MAX_STACK];
@@ -387,26 +420,41 @@ extern "C" int frk_filter_eval(const frk_filter_query* d_queries, uint32_t n_que
         frk::SetError("frk_filter_eval: grid limit (segments %u, queries %u)", n_segments, n_queries);
         return FRK_ERR_UNSUPPORTED;
     }
-    // W words per thread: as many as keep every operand of the batch in an LDS slot (S = 16 / W),
-    // and no more than the segment needs.
-    const uint32_t need = max_postings == 0 ? 16u : max_postings;
+    // W words per thread: the most that still give every operand of the batch's largest program an
+    // LDS slot (16 / W slots; all 16 when unknown), and no more than the segment needs. Whole-segment
+    // chunks (W = 4 at 131,072 docs) spare the sparse operands their range search. S slots: one per
+    // operand, so the LDS shrinks with the programs.
+    const uint32_t need = max_postings == 0 || max_postings > 16 ? 16u : max_postings;
     uint32_t w = need <= 4 ? 4u : need <= 8 ? 2u : 1u;
     while (w > 1 && static_cast<uint32_t>(frk::kThreads) * (w / 2) >= seg_words) w /= 2;
+    const uint32_t slots = need < 16u / w ? need : 16u / w;
     const uint32_t chunk = frk::kThreads * w;
+    const size_t lds = static_cast<size_t>(slots) * chunk * sizeof(uint64_t);
     dim3 grid((seg_words + chunk - 1) / chunk, n_segments, n_queries);
     hipStream_t st = static_cast<hipStream_t>(stream);
+    static bool lds_set = false;  // allow 64 KiB of dynamic LDS for every instantiation, once
+    if (!lds_set) {
+        const int max_lds = frk::kMaxSlots * frk::kThreads * static_cast<int>(sizeof(uint64_t));
+        FRK_HIP_CHECK(hipFuncSetAttribute(reinterpret_cast<const void*>(&frk::FilterKernel<4>),
+                                          hipFuncAttributeMaxDynamicSharedMemorySize, max_lds));
+        FRK_HIP_CHECK(hipFuncSetAttribute(reinterpret_cast<const void*>(&frk::FilterKernel<2>),
+                                          hipFuncAttributeMaxDynamicSharedMemorySize, max_lds));
+        FRK_HIP_CHECK(hipFuncSetAttribute(reinterpret_cast<const void*>(&frk::FilterKernel<1>),
+                                          hipFuncAttributeMaxDynamicSharedMemorySize, max_lds));
+        lds_set = true;
+    }
     switch (w) {
         case 4:
-            hipLaunchKernelGGL(frk::FilterKernel<4>, grid, dim3(frk::kThreads), 0, st, d_queries, d_expr, d_table,
-                               seg_docs, n_docs, d_out, out_stride, d_counts, d_status);
+            hipLaunchKernelGGL(frk::FilterKernel<4>, grid, dim3(frk::kThreads), lds, st, d_queries, d_expr, d_table,
+                               seg_docs, n_docs, d_out, out_stride, d_counts, d_status, slots);
             break;
         case 2:
-            hipLaunchKernelGGL(frk::FilterKernel<2>, grid, dim3(frk::kThreads), 0, st, d_queries, d_expr, d_table,
-                               seg_docs, n_docs, d_out, out_stride, d_counts, d_status);
+            hipLaunchKernelGGL(frk::FilterKernel<2>, grid, dim3(frk::kThreads), lds, st, d_queries, d_expr, d_table,
+                               seg_docs, n_docs, d_out, out_stride, d_counts, d_status, slots);
             break;
         default:
-            hipLaunchKernelGGL(frk::FilterKernel<1>, grid, dim3(frk::kThreads), 0, st, d_queries, d_expr, d_table,
-                               seg_docs, n_docs, d_out, out_stride, d_counts, d_status);
+            hipLaunchKernelGGL(frk::FilterKernel<1>, grid, dim3(frk::kThreads), lds, st, d_queries, d_expr, d_table,
+                               seg_docs, n_docs, d_out, out_stride, d_counts, d_status, slots);
             break;
     }
     FRK_HIP_CHECK(hipGetLastError());
