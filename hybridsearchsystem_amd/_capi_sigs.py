@@ -40,6 +40,8 @@ def bind(L):
     L.frk_kth_largest.argtypes = [vp, u32, u32, u32, vp, u32, vp, vp]
     L.frk_rescore.restype = i32
     L.frk_rescore.argtypes = [vp, vp, u32, vp, u32, vp, vp, vp]
+    L.frk_rescore_rows.restype = i32
+    L.frk_rescore_rows.argtypes = [vp, u32, u64, vp, u32, vp, u32, vp, vp, vp]
     L.frk_table_load.restype = i32
     L.frk_table_load.argtypes = [i32, cp, u32, ctypes.POINTER(vp)]
     L.frk_table_free.restype = None

@@ -118,6 +118,7 @@ VectorFieldData::~VectorFieldData() {
     for (void* p : m_tiles)
         if (p) (void)hipFree(p);
     if (m_scales) (void)hipFree(m_scales);
+    if (m_rows) (void)hipFree(m_rows);
     if (m_staging) (void)hipFree(m_staging);
 }
 
@@ -145,6 +146,15 @@ bool VectorFieldData::Allocate(int32_t deviceId, uint32_t dtype_mask) {
         m_store[dt].dim = dp;
         m_store[dt].n_docs = m_docNum;
     }
+    if (dtype_mask & ((1u << FRK_DTYPE_BF16) | (1u << FRK_DTYPE_FP8E4M3))) {
+        m_rowsBytes = m_docNum * m_store[FRK_DTYPE_F32].dim * sizeof(float);
+        if (hipMalloc(reinterpret_cast<void**>(&m_rows), m_rowsBytes + 16) != hipSuccess) {
+            FRK_LOG_ERROR("hipMalloc " << m_rowsBytes << " B for the row-major FP32 copy of " << m_name);
+            m_rows = nullptr;
+            return false;
+        }
+        if (hipMemset(m_rows, 0, m_rowsBytes + 16) != hipSuccess) return false;  // the padding dims stay 0
+    }
     return true;
 }
 
@@ -159,6 +169,14 @@ bool VectorFieldData::SetRows(const float* rows, uint64_t first, uint64_t n, boo
             if (!m_tiles[dt]) continue;
             if (frk_tile_rows_device(d_rows, cnt, m_dim, at, dt, m_store[dt].dim, m_tiles[dt], m_scales, s) != 0) {
                 FRK_LOG_ERROR("tiling failed: " << frk_last_error());
+                return false;
+            }
+        }
+        if (m_rows) {
+            const uint32_t dp = m_store[FRK_DTYPE_F32].dim;
+            if (hipMemcpy2DAsync(m_rows + at * dp, dp * sizeof(float), d_rows, m_dim * sizeof(float),
+                                 m_dim * sizeof(float), cnt, hipMemcpyDeviceToDevice, s) != hipSuccess) {
+                FRK_LOG_ERROR("row copy failed for " << m_name);
                 return false;
             }
         }
@@ -189,7 +207,7 @@ const frk_doc_store* VectorFieldData::Store(uint32_t dtype) const {
 }
 
 uint64_t VectorFieldData::DeviceBytes() const {
-    return m_bytes[0] + m_bytes[1] + m_bytes[2] + (m_scales ? m_docNum * 4 : 0);
+    return m_bytes[0] + m_bytes[1] + m_bytes[2] + (m_scales ? m_docNum * 4 : 0) + m_rowsBytes;
 }
 
 // ------------------------------------------------------------------------------------------ table

@@ -65,7 +65,8 @@ class PostingFieldData {
 };
 
 // One vector field: FP32 tiles (the plain path) plus optional BF16 and FP8+scale tiles, all in the
-// MFMA-native doc tiling (frk_capi.h). Replaces VectorFieldData's ZN FP16 body
+// MFMA-native doc tiling (frk_capi.h), and with a low-precision store a row-major FP32 copy that the
+// FP32 rescoring reads (one contiguous row per candidate). Replaces VectorFieldData's ZN FP16 body
 // (index/vector/vector_field_data.cpp:10-75).
 class VectorFieldData {
    public:
@@ -80,6 +81,8 @@ class VectorFieldData {
     // `rows` is host or device memory ([n][dim] FP32).
     bool SetRows(const float* rows, uint64_t first, uint64_t n, bool rows_on_device, void* stream);
     const frk_doc_store* Store(uint32_t dtype) const;
+    // Row-major FP32 rows, row stride PaddedDim(FRK_DTYPE_F32) (zero padding), or null.
+    const float* RowsF32() const { return m_rows; }
     uint32_t Dimension() const { return m_dim; }
     uint32_t PaddedDim(uint32_t dtype) const;
     uint64_t DocNum() const { return m_docNum; }
@@ -93,6 +96,8 @@ class VectorFieldData {
     frk_doc_store m_store[3] = {};
     void* m_tiles[3] = {nullptr, nullptr, nullptr};
     float* m_scales = nullptr;
+    float* m_rows = nullptr;
+    uint64_t m_rowsBytes = 0;
     float* m_staging = nullptr;  // device rows staging for host uploads
     uint64_t m_stagingRows = 0;
     uint64_t m_bytes[3] = {0, 0, 0};

@@ -221,6 +221,7 @@ bool FullRecallSearcher::RunDevice(const VectorFieldData& vf, std::vector<AggrDe
     const uint32_t dtype = m_options.dtype;
     const frk_doc_store* store = vf.Store(dtype);
     const frk_doc_store* f32 = vf.Store(FRK_DTYPE_F32);
+    const float* rowsF32 = vf.RowsF32();
     if (!store || !f32) {
         FRK_LOG_ERROR("vector store for dtype " << dtype << " was not built for field " << vf.Name());
         return false;
@@ -461,8 +462,12 @@ bool FullRecallSearcher::RunDevice(const VectorFieldData& vf, std::vector<AggrDe
                                 dCnt + cLp, nullptr, s) != 0)
                 return "select k*m";
             rec(4);
-            if (frk_rescore(f32, dQF32, Q, ws->lp.As<uint64_t>(), kSel, dCnt + cLp, ws->rescored.As<uint64_t>(), s) != 0)
-                return "rescore";
+            // from the row-major FP32 copy when the field keeps one (one contiguous read per candidate)
+            const int rc = rowsF32 ? frk_rescore_rows(rowsF32, dpF32, nDocs, dQF32, Q, ws->lp.As<uint64_t>(), kSel,
+                                                      dCnt + cLp, ws->rescored.As<uint64_t>(), s)
+                                   : frk_rescore(f32, dQF32, Q, ws->lp.As<uint64_t>(), kSel, dCnt + cLp,
+                                                 ws->rescored.As<uint64_t>(), s);
+            if (rc != 0) return "rescore";
             if (frk_select_topk(ws->rescored.As<uint64_t>(), kSel, dCnt + cLp, Q, dK, 0, ws->out.As<uint64_t>(), kmax,
                                 dCnt + cOut, nullptr, s) != 0)
                 return "select after rescore";
@@ -481,7 +486,8 @@ bool FullRecallSearcher::RunDevice(const VectorFieldData& vf, std::vector<AggrDe
     } else {
         // every scalar and address the sequence bakes into its launches
         const uint64_t key[] = {Q, kmax, kSel, m, dtype, reinterpret_cast<uint64_t>(store->tiles),
-                                reinterpret_cast<uint64_t>(f32->tiles), nDocs, segs, dnps, useProbe, probeStep,
+                                reinterpret_cast<uint64_t>(f32->tiles), reinterpret_cast<uint64_t>(rowsF32), nDocs,
+                                segs, dnps, useProbe, probeStep,
                                 probeBlocks, learned, cap, mpLaunch, batchBytes, oExpr, oTab, callerFloor, qStride,
                                 reinterpret_cast<uint64_t>(ws->batch.p), reinterpret_cast<uint64_t>(ws->hbatch.p),
                                 reinterpret_cast<uint64_t>(ws->bitmaps.p), reinterpret_cast<uint64_t>(ws->bmasks.p),

@@ -35,6 +35,48 @@ __global__ __launch_bounds__(256) void RescoreKernel(const floatx4* __restrict__
     if (lane == 0) keys_out[static_cast<uint64_t>(q) * stride + i] = MakeKey(acc, doc);
 }
 
+// Rescoring from row-major rows (row stride dim, a multiple of 4): one wave per four candidates of a
+// query. Lane l covers dims [4 l + 256 r, +4), so a candidate's row is read with contiguous 16-byte
+// loads (1 KiB per wave instruction at 256 dims), and the four candidates' loads are all issued before
+// any of them is summed.
+__global__ __launch_bounds__(256) void RescoreRowsKernel(const float* __restrict__ rows, uint32_t dim, uint64_t n_docs,
+                                                         const float* __restrict__ queries,
+                                                         const uint64_t* __restrict__ keys_in, uint32_t stride,
+                                                         const uint32_t* __restrict__ counts,
+                                                         uint64_t* __restrict__ keys_out) {
+    constexpr int kPer = 4;  // candidates per wave
+    const uint32_t q = blockIdx.x;
+    const uint32_t lane = threadIdx.x & 63;
+    const uint32_t i0 = (blockIdx.y * 4 + (threadIdx.x >> 6)) * kPer;
+    const uint32_t n = counts[q] < stride ? counts[q] : stride;
+    if (i0 >= n) return;
+    const float* qv = queries + static_cast<uint64_t>(q) * dim;
+    uint32_t doc[kPer];
+    bool ok[kPer];
+#pragma unroll
+    for (int c = 0; c < kPer; c++) {
+        ok[c] = i0 + c < n;
+        doc[c] = ok[c] ? KeyDoc(keys_in[static_cast<uint64_t>(q) * stride + i0 + c]) : 0u;
+        ok[c] = ok[c] && doc[c] < n_docs;
+    }
+    float acc[kPer] = {0.0f, 0.0f, 0.0f, 0.0f};
+    for (uint32_t d = 4 * lane; d < dim; d += 256) {
+        const floatx4 qq = *reinterpret_cast<const floatx4*>(qv + d);
+        floatx4 v[kPer];
+#pragma unroll
+        for (int c = 0; c < kPer; c++)
+            v[c] = ok[c] ? *reinterpret_cast<const floatx4*>(rows + static_cast<uint64_t>(doc[c]) * dim + d)
+                         : floatx4{0.0f, 0.0f, 0.0f, 0.0f};
+#pragma unroll
+        for (int c = 0; c < kPer; c++) acc[c] += v[c][0] * qq[0] + v[c][1] * qq[1] + v[c][2] * qq[2] + v[c][3] * qq[3];
+    }
+#pragma unroll
+    for (int c = 0; c < kPer; c++)
+        for (int off = 32; off > 0; off >>= 1) acc[c] += __shfl_xor(acc[c], off, 64);
+    if (lane < kPer && ok[lane])
+        keys_out[static_cast<uint64_t>(q) * stride + i0 + lane] = MakeKey(acc[lane], doc[lane]);
+}
+
 }  // namespace
 }  // namespace frk
 
@@ -55,6 +97,26 @@ extern "C" int frk_rescore(const frk_doc_store* f32_store, const float* d_querie
     hipLaunchKernelGGL(frk::RescoreKernel, grid, dim3(256), 0, static_cast<hipStream_t>(stream),
                        static_cast<const frk::floatx4*>(f32_store->tiles), chunks, d_queries, f32_store->dim,
                        d_keys_in, stride, d_counts, d_keys_out);
+    FRK_HIP_CHECK(hipGetLastError());
+    return FRK_OK;
+}
+
+extern "C" int frk_rescore_rows(const float* d_rows, uint32_t dim, uint64_t n_docs, const float* d_queries,
+                                uint32_t n_queries, const uint64_t* d_keys_in, uint32_t stride, const uint32_t* d_counts,
+                                uint64_t* d_keys_out, frk_stream_t stream) {
+    if (!d_rows || !d_queries || !d_keys_in || !d_counts || !d_keys_out || dim == 0 || dim % 4 != 0 ||
+        (reinterpret_cast<uintptr_t>(d_rows) & 15) != 0 || (reinterpret_cast<uintptr_t>(d_queries) & 15) != 0) {
+        frk::SetError("frk_rescore_rows: needs 16-byte aligned rows and queries with a dim that is a multiple of 4");
+        return FRK_ERR_INVALID;
+    }
+    if (n_queries == 0 || stride == 0) return FRK_OK;
+    dim3 grid(n_queries, (stride + 15) / 16);
+    if (grid.y > 65535) {
+        frk::SetError("frk_rescore_rows: stride %u too large", stride);
+        return FRK_ERR_UNSUPPORTED;
+    }
+    hipLaunchKernelGGL(frk::RescoreRowsKernel, grid, dim3(256), 0, static_cast<hipStream_t>(stream), d_rows, dim,
+                       n_docs, d_queries, d_keys_in, stride, d_counts, d_keys_out);
     FRK_HIP_CHECK(hipGetLastError());
     return FRK_OK;
 }

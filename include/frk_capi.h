@@ -180,6 +180,12 @@ int frk_select_topk(const uint64_t* d_cand, uint32_t cand_cap, const uint32_t* d
  * ------------------------------------------------------------------------------------------------ */
 int frk_rescore(const frk_doc_store* f32_store, const float* d_queries, uint32_t n_queries, const uint64_t* d_keys_in,
                 uint32_t stride, const uint32_t* d_counts, uint64_t* d_keys_out, frk_stream_t stream);
+/* The same from row-major FP32 rows d_rows[doc * dim ..] (dim a multiple of 4, rows 16-byte aligned):
+ * each candidate's row is one contiguous read instead of one 16-byte piece per tile. d_queries:
+ * [n_queries][dim]. */
+int frk_rescore_rows(const float* d_rows, uint32_t dim, uint64_t n_docs, const float* d_queries, uint32_t n_queries,
+                     const uint64_t* d_keys_in, uint32_t stride, const uint32_t* d_counts, uint64_t* d_keys_out,
+                     frk_stream_t stream);
 
 /* ------------------------------------------------------------------------------------------------
  * Learned score floor: BF16 training and inference kernels of a small MLP

@@ -288,6 +288,50 @@ def test_low_precision_scores_match_dequantized_values(frk, dtype):
         assert np.all(np.abs(sc - exact) <= 2e-5 * scale + 1e-30), (q, np.max(np.abs(sc - exact) / scale))
 
 
+def test_rescore_from_tiles_and_rows(frk):
+    """FP32 rescoring of candidate keys, from the tiled store and from row-major rows: both give each
+    candidate its doc and the FP32 dot product within 1e-5 of sum|q_d v_d| (FP64 truth); counts past the
+    stride and doc ids out of the table are respected."""
+    import ctypes
+    from hybridsearchsystem_amd import check, lib
+    rng = np.random.default_rng(71)
+    n, dim, nq, stride = 5000, 100, 9, 37
+    docs = rng.standard_normal((n, dim)).astype(np.float32)
+    qs = rng.standard_normal((nq, dim)).astype(np.float32)
+    tiles, _, dp = frk.tile_docs(docs, 0)
+    rows = np.zeros((n, dp), np.float32)
+    rows[:, :dim] = docs
+    qp = np.zeros((nq, dp), np.float32)
+    qp[:, :dim] = qs
+    ids = np.stack([rng.choice(n, stride, replace=False) for _ in range(nq)]).astype(np.uint32)
+    keys = (np.uint64(0x12345678) << np.uint64(32)) | (np.uint64(0xFFFFFFFF) - ids.astype(np.uint64))
+    counts = rng.integers(0, stride + 5, nq).astype(np.uint32)
+    counts[0] = stride + 100  # more than the stride holds: the first `stride` are rescored
+    d_t, d_r, d_q = dev(tiles), dev(rows), dev(qp)
+    d_k, d_c = dev(keys.view(np.int64)), dev(counts.view(np.int32))
+    store = frk.doc_store(d_t, None, 0, dp, n)
+    outs = []
+    for use_rows in (False, True):
+        d_o = torch.zeros((nq, stride), dtype=torch.int64, device="cuda")
+        if use_rows:
+            check(lib().frk_rescore_rows(d_r.data_ptr(), dp, n, d_q.data_ptr(), nq, d_k.data_ptr(), stride,
+                                         d_c.data_ptr(), d_o.data_ptr(), None), "rescore_rows")
+        else:
+            check(lib().frk_rescore(ctypes.byref(store), d_q.data_ptr(), nq, d_k.data_ptr(), stride, d_c.data_ptr(),
+                                    d_o.data_ptr(), None), "rescore")
+        torch.cuda.synchronize()
+        outs.append(d_o.cpu().numpy().view(np.uint64))
+    d64, q64 = docs.astype(np.float64), qs.astype(np.float64)
+    for out in outs:
+        for q in range(nq):
+            m = min(int(counts[q]), stride)
+            sc, got = frk.decode_keys(out[q, :m])
+            assert np.array_equal(got, ids[q, :m].astype(np.int64))
+            exact = d64[got] @ q64[q]
+            assert np.all(np.abs(sc - exact) <= 1e-5 * np.abs(d64[got] * q64[q]).sum(axis=1) + 1e-30)
+            assert np.all(out[q, m:] == 0)
+
+
 def test_fp32_ties_broken_by_doc_id(frk):
     rng = np.random.default_rng(21)
     n, dim, nq = 3000, 64, 8
