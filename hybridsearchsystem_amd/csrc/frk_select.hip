@@ -3,9 +3,10 @@
 // The reference pre-filters with a single-core sampled-pivot partition (device/topk/kernel_topk.h:30-388),
 // returns a superset of the top-k and finishes with a host nth_element + sort
 // (searcher/stage/result_aggregator.cpp:993-1000). Here the whole selection stays on the device:
-// an MSB-first radix select over the 64-bit keys (8 passes of 8-bit digits, histograms in LDS) finds
-// the k-th largest key T exactly -- keys are unique because the low word is the doc id -- then the k
-// keys >= T are gathered into LDS and bitonic-sorted descending. Because the key orders by score and
+// up to FRK_SELECT_MAX_K candidates are bitonic-sorted in LDS outright; more go through an MSB-first
+// radix select over the 64-bit keys (8 passes of 8-bit digits, histograms in LDS, the keys in LDS when
+// they fit) that finds the k-th largest key T exactly -- keys are unique because the low word is the
+// doc id -- then the k keys >= T are gathered into LDS and bitonic-sorted descending. Because the key orders by score and
 // then by ascending doc id, the result is the top-k under the reference CPU truth's ordering
 // (10.13/load/src/harness/recall/recall_ref.cpp:19-25).
 #include <math.h>
@@ -16,6 +17,8 @@ namespace frk {
 namespace {
 
 constexpr int kThreads = 1024;
+constexpr uint32_t kLdsKeys = 8192;     // candidate keys a workgroup keeps in LDS for the radix passes
+constexpr uint32_t kLdsVals = 16384;    // values the k-th largest kernel keeps in LDS
 
 __device__ inline void BitonicSortDesc(uint64_t* s, uint32_t n_pow2) {
     for (uint32_t size = 2; size <= n_pow2; size <<= 1) {
@@ -35,22 +38,53 @@ __device__ inline void BitonicSortDesc(uint64_t* s, uint32_t n_pow2) {
     }
 }
 
+// One radix pass's verdict, by the first wave: the digit (bin) that holds the rem-th largest element
+// of a 256-bin histogram counted from the top bin, and how many elements the bins above it hold.
+// Lane l sums bins 255 - 4l .. 252 - 4l, a wave prefix sum over the lanes finds the lane whose bins
+// cross rem, and that lane walks its four bins. Needs rem <= the histogram's total.
+__device__ inline void FindDigit(const uint32_t* hist, uint32_t rem, uint32_t* digit, uint32_t* above) {
+    const int lane = threadIdx.x & 63;
+    uint32_t c[4], sum = 0;
+#pragma unroll
+    for (int b = 0; b < 4; b++) {
+        c[b] = hist[255 - 4 * lane - b];
+        sum += c[b];
+    }
+    uint32_t incl = sum;
+#pragma unroll
+    for (int off = 1; off < 64; off <<= 1) {
+        const uint32_t o = __shfl_up(incl, off, 64);
+        if (lane >= off) incl += o;
+    }
+    const uint32_t excl = incl - sum;
+    if (excl < rem && rem <= incl) {
+        uint32_t cum = excl;
+        int b = 0;
+        for (; b < 3; b++) {
+            if (cum + c[b] >= rem) break;
+            cum += c[b];
+        }
+        *digit = static_cast<uint32_t>(255 - 4 * lane - b);
+        *above = cum;
+    }
+}
+
 __global__ __launch_bounds__(kThreads) void SelectKernel(const uint64_t* __restrict__ cand, uint32_t cap,
                                                          const uint32_t* __restrict__ counts,
                                                          const uint32_t* __restrict__ ks, uint32_t k_default,
                                                          uint64_t* __restrict__ out, uint32_t k_stride,
                                                          uint32_t* __restrict__ out_counts, float* __restrict__ kth) {
     __shared__ uint32_t hist[256];
-    __shared__ uint64_t s_prefix;
-    __shared__ uint32_t s_remaining;
-    __shared__ uint32_t s_fill;
+    __shared__ uint32_t s_digit, s_above, s_fill;
     __shared__ uint64_t sorted[FRK_SELECT_MAX_K];
+    __shared__ uint64_t keys[kLdsKeys];
 
     const uint32_t q = blockIdx.x;
     const uint32_t n = counts[q] < cap ? counts[q] : cap;
     const uint32_t kq = ks ? (ks[q] < k_stride ? ks[q] : k_stride) : k_default;
     const uint32_t k = kq < n ? kq : n;
     const uint64_t* c = cand + static_cast<uint64_t>(q) * cap;
+    uint64_t* o = out + static_cast<uint64_t>(q) * k_stride;
     if (k == 0) {
         if (threadIdx.x == 0) {
             out_counts[q] = 0;
@@ -58,49 +92,64 @@ __global__ __launch_bounds__(kThreads) void SelectKernel(const uint64_t* __restr
         }
         return;
     }
-    if (threadIdx.x == 0) {
-        s_prefix = 0;
-        s_remaining = k;
-        s_fill = 0;
+    if (n <= FRK_SELECT_MAX_K) {
+        // few candidates: sort them all (padding keys 0 sort last)
+        uint32_t p2 = 1;
+        while (p2 < n) p2 <<= 1;
+        for (uint32_t i = threadIdx.x; i < p2; i += kThreads) sorted[i] = i < n ? c[i] : 0ull;
+        __syncthreads();
+        BitonicSortDesc(sorted, p2);
+        for (uint32_t i = threadIdx.x; i < k; i += kThreads) o[i] = sorted[i];
+        if (threadIdx.x == 0) {
+            out_counts[q] = k;
+            if (kth) kth[q] = (kq <= n) ? KeyScore(sorted[k - 1]) : -INFINITY;
+        }
+        return;
     }
-    uint64_t prefix_mask = 0;
+    // MSB-first radix select of the k-th largest key (8 passes of 8-bit digits; keys are unique), over
+    // an LDS copy of the keys when they fit
+    const bool in_lds = n <= kLdsKeys;
+    if (in_lds)
+        for (uint32_t i = threadIdx.x; i < n; i += kThreads) keys[i] = c[i];
+    if (threadIdx.x == 0) s_fill = 0;
+    uint64_t prefix = 0, pmask = 0;
+    uint32_t rem = k;
     for (int shift = 56; shift >= 0; shift -= 8) {
         for (int b = threadIdx.x; b < 256; b += kThreads) hist[b] = 0;
         __syncthreads();
-        const uint64_t prefix = s_prefix;
-        for (uint32_t i = threadIdx.x; i < n; i += kThreads) {
-            const uint64_t key = c[i];
-            if ((key & prefix_mask) == prefix) atomicAdd(&hist[(key >> shift) & 255u], 1u);
-        }
-        __syncthreads();
-        if (threadIdx.x == 0) {
-            uint32_t rem = s_remaining, cum = 0;
-            int d = 255;
-            for (; d > 0; d--) {
-                if (cum + hist[d] >= rem) break;
-                cum += hist[d];
+        // defaults, never used while rem <= the count (set after every thread read the last verdict)
+        if (threadIdx.x == 0) s_digit = s_above = 0;
+        auto pass = [&](const uint64_t* src) __attribute__((always_inline)) {
+            for (uint32_t i = threadIdx.x; i < n; i += kThreads) {
+                const uint64_t key = src[i];
+                if ((key & pmask) == prefix) atomicAdd(&hist[(key >> shift) & 255u], 1u);
             }
-            s_remaining = rem - cum;
-            s_prefix = prefix | (static_cast<uint64_t>(d) << shift);
-        }
-        prefix_mask |= 0xFFull << shift;
+        };
+        if (in_lds) pass(keys); else pass(c);
         __syncthreads();
+        if (threadIdx.x < 64) FindDigit(hist, rem, &s_digit, &s_above);
+        __syncthreads();
+        prefix |= static_cast<uint64_t>(s_digit) << shift;
+        rem -= s_above;
+        pmask |= 0xFFull << shift;
     }
-    const uint64_t T = s_prefix;  // the k-th largest key
-    for (uint32_t i = threadIdx.x; i < n; i += kThreads) {
-        const uint64_t key = c[i];
-        if (key >= T) {
-            const uint32_t at = atomicAdd(&s_fill, 1u);
-            if (at < FRK_SELECT_MAX_K) sorted[at] = key;
+    const uint64_t T = prefix;  // the k-th largest key
+    auto gather = [&](const uint64_t* src) __attribute__((always_inline)) {
+        for (uint32_t i = threadIdx.x; i < n; i += kThreads) {
+            const uint64_t key = src[i];
+            if (key >= T) {
+                const uint32_t at = atomicAdd(&s_fill, 1u);
+                if (at < FRK_SELECT_MAX_K) sorted[at] = key;
+            }
         }
-    }
+    };
+    if (in_lds) gather(keys); else gather(c);
     __syncthreads();
     uint32_t p2 = 1;
     while (p2 < k) p2 <<= 1;
     for (uint32_t i = k + threadIdx.x; i < p2; i += kThreads) sorted[i] = 0;
     __syncthreads();
     BitonicSortDesc(sorted, p2);
-    uint64_t* o = out + static_cast<uint64_t>(q) * k_stride;
     for (uint32_t i = threadIdx.x; i < k; i += kThreads) o[i] = sorted[i];
     if (threadIdx.x == 0) {
         out_counts[q] = k;
@@ -108,12 +157,14 @@ __global__ __launch_bounds__(kThreads) void SelectKernel(const uint64_t* __restr
     }
 }
 
-// k-th largest of n floats per query (radix select over order-preserving u32 keys, 4 passes).
+// k-th largest of n floats per query (radix select over order-preserving u32 keys, 4 passes, over an
+// LDS copy of the values when they fit).
 __global__ __launch_bounds__(kThreads) void KthLargestKernel(const float* __restrict__ vals, uint32_t stride,
                                                              uint32_t n, const uint32_t* __restrict__ ks,
                                                              uint32_t k_default, float* __restrict__ out) {
     __shared__ uint32_t hist[256];
-    __shared__ uint32_t s_prefix, s_remaining;
+    __shared__ uint32_t s_digit, s_above;
+    __shared__ uint32_t okeys[kLdsVals];
     const uint32_t q = blockIdx.x;
     const uint32_t k = ks ? ks[q] : k_default;
     const float* v = vals + static_cast<uint64_t>(q) * stride;
@@ -121,34 +172,26 @@ __global__ __launch_bounds__(kThreads) void KthLargestKernel(const float* __rest
         if (threadIdx.x == 0) out[q] = -INFINITY;
         return;
     }
-    if (threadIdx.x == 0) {
-        s_prefix = 0;
-        s_remaining = k;
-    }
-    uint32_t mask = 0;
+    const bool in_lds = n <= kLdsVals;
+    if (in_lds)
+        for (uint32_t i = threadIdx.x; i < n; i += kThreads) okeys[i] = OrderedFromFloat(v[i]);
+    uint32_t prefix = 0, mask = 0, rem = k;
     for (int shift = 24; shift >= 0; shift -= 8) {
         for (int b = threadIdx.x; b < 256; b += kThreads) hist[b] = 0;
         __syncthreads();
-        const uint32_t prefix = s_prefix;
+        if (threadIdx.x == 0) s_digit = s_above = 0;
         for (uint32_t i = threadIdx.x; i < n; i += kThreads) {
-            const uint32_t key = OrderedFromFloat(v[i]);
+            const uint32_t key = in_lds ? okeys[i] : OrderedFromFloat(v[i]);
             if ((key & mask) == prefix) atomicAdd(&hist[(key >> shift) & 255u], 1u);
         }
         __syncthreads();
-        if (threadIdx.x == 0) {
-            uint32_t rem = s_remaining, cum = 0;
-            int d = 255;
-            for (; d > 0; d--) {
-                if (cum + hist[d] >= rem) break;
-                cum += hist[d];
-            }
-            s_remaining = rem - cum;
-            s_prefix = prefix | (static_cast<uint32_t>(d) << shift);
-        }
-        mask |= 0xFFu << shift;
+        if (threadIdx.x < 64) FindDigit(hist, rem, &s_digit, &s_above);
         __syncthreads();
+        prefix |= s_digit << shift;
+        rem -= s_above;
+        mask |= 0xFFu << shift;
     }
-    if (threadIdx.x == 0) out[q] = FloatFromOrdered(s_prefix);
+    if (threadIdx.x == 0) out[q] = FloatFromOrdered(prefix);
 }
 
 }  // namespace

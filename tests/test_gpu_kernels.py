@@ -332,6 +332,55 @@ def test_rescore_from_tiles_and_rows(frk):
             assert np.all(out[q, m:] == 0)
 
 
+def test_select_and_kth_largest_at_every_path_boundary(frk):
+    """Top-k select against numpy on unique keys at candidate counts around the paths: sorted outright
+    (<= 4096), radix passes over an LDS copy (<= 8192), radix passes over HBM; k above and below n,
+    counts above the capacity. The k-th largest float likewise around its LDS size (16384)."""
+    from hybridsearchsystem_amd import check, lib
+    rng = np.random.default_rng(81)
+    sizes = [0, 1, 100, 4095, 4096, 4097, 8192, 8193, 20000]
+    cap = max(sizes)
+    nq = len(sizes)
+    keys = np.zeros((nq, cap), np.uint64)
+    for q, n in enumerate(sizes):
+        sc = rng.standard_normal(n).astype(np.float32)
+        sc[: n // 7] = 0.5  # ties in score, broken by doc id
+        docs = rng.choice(1 << 31, n, replace=False).astype(np.uint64)
+        u = sc.view(np.uint32).astype(np.uint64)
+        ordered = np.where(u >> np.uint64(31), ~u & np.uint64(0xFFFFFFFF), u | np.uint64(0x80000000))
+        keys[q, :n] = (ordered << np.uint64(32)) | (np.uint64(0xFFFFFFFF) - docs)
+    counts = np.array(sizes, np.uint32)
+    counts[-1] = cap + 50  # more than the capacity: only `cap` keys exist
+    for kv in (1, 7, 100, 1000, 4096):
+        ks = np.full(nq, kv, np.int32)
+        d_c, d_n, d_k = dev(keys.view(np.int64)), dev(counts.view(np.int32)), dev(ks)
+        d_out = torch.zeros((nq, kv), dtype=torch.int64, device="cuda")
+        d_oc = torch.zeros(nq, dtype=torch.int32, device="cuda")
+        d_kth = torch.zeros(nq, dtype=torch.float32, device="cuda")
+        frk.select_topk(d_c, cap, d_n, nq, d_k, 0, d_out, kv, d_oc, d_kth)
+        torch.cuda.synchronize()
+        out, oc, kth = d_out.cpu().numpy().view(np.uint64), d_oc.cpu().numpy(), d_kth.cpu().numpy()
+        for q, n in enumerate(sizes):
+            want = np.sort(keys[q, :n])[::-1][:kv]
+            assert oc[q] == len(want), (kv, n)
+            assert np.array_equal(out[q, :oc[q]], want), (kv, n)
+            if kv <= n:
+                assert kth[q] == frk.decode_keys(want[-1:])[0][0]
+            else:
+                assert kth[q] == -np.inf
+    for n in (100, 16384, 16385, 40000):
+        vals = rng.standard_normal((3, n)).astype(np.float32)
+        vals[1, : n // 3] = 0.25
+        ks = np.array([1, n // 2, n], np.uint32)
+        d_v, d_k = dev(vals), dev(ks.view(np.int32))
+        d_o = torch.zeros(3, dtype=torch.float32, device="cuda")
+        check(lib().frk_kth_largest(d_v.data_ptr(), n, n, 3, d_k.data_ptr(), 0, d_o.data_ptr(), None), "kth")
+        torch.cuda.synchronize()
+        got = d_o.cpu().numpy()
+        for q in range(3):
+            assert got[q] == np.sort(vals[q])[::-1][ks[q] - 1], (n, q)
+
+
 def test_fp32_ties_broken_by_doc_id(frk):
     rng = np.random.default_rng(21)
     n, dim, nq = 3000, 64, 8
