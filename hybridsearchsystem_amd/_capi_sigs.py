@@ -13,7 +13,8 @@ class SearchOptions(ctypes.Structure):
 class SearchStats(ctypes.Structure):
     _fields_ = [("ms_filter", f32), ("ms_probe", f32), ("ms_score", f32), ("ms_select", f32), ("ms_rescore", f32),
                 ("ms_total", f32), ("candidates", u64), ("reruns", u32), ("overflowed", u32), ("probe_step", u32),
-                ("cand_cap", u32), ("graph_replayed", u32)]
+                ("cand_cap", u32), ("graph_replayed", u32), ("ms_host_prepare", f32), ("ms_host_finish", f32),
+                ("ms_call", f32)]
 
 
 class IndexVectorField(ctypes.Structure):
@@ -82,6 +83,13 @@ def bind(L):
     L.frk_fill_u32.argtypes = [vp, u64, u32, vp]
     L.frk_floor_features.restype = i32
     L.frk_floor_features.argtypes = [vp, cp, u32, vp, vp, vp, u32, vp]
+    L.frk_filter_prepare.restype = i32
+    L.frk_filter_prepare.argtypes = [cp, cp, ctypes.POINTER(vp)]
+    L.frk_filter_free.restype = None
+    L.frk_filter_free.argtypes = [vp]
+    L.frk_search_prepared.restype = i32
+    L.frk_search_prepared.argtypes = [vp, cp, u32, vp, u32, vp, vp, ctypes.POINTER(SearchOptions), u32, vp, vp, vp,
+                                      vp, ctypes.POINTER(SearchStats)]
     L.frk_search.restype = i32
     L.frk_search.argtypes = [vp, cp, cp, u32, vp, u32, vp, vp, ctypes.POINTER(SearchOptions), u32, vp, vp, vp, vp,
                              ctypes.POINTER(SearchStats)]

@@ -2,12 +2,14 @@
 // and building, and batch search with filter strings, for callers that cannot link C++.
 #include <hip/hip_runtime.h>
 
+#include <chrono>
 #include <cstring>
 #include <memory>
 #include <string>
 #include <vector>
 
 #include "../../../include/frk_capi.h"
+#include "../frk_common.h"
 #include "data_table.h"
 #include "full_recall_searcher.h"
 #include "index_writer.h"
@@ -27,6 +29,10 @@ struct frk_builder {
 
 struct frk_server {
     std::unique_ptr<SearchServer> s;
+};
+
+struct frk_filter {
+    std::unique_ptr<QueryNode> tree;  // built once; adjusted on first use, then shared read-only
 };
 
 namespace {
@@ -231,27 +237,17 @@ int frk_floor_features(const frk_table* t, const char* posting_field, uint32_t n
     return FRK_OK;
 }
 
-int frk_search(const frk_table* t, const char* vector_field, const char* posting_field, uint32_t n_queries,
-               const float* queries, uint32_t dim, const char* const* filters, const uint32_t* ks,
-               const frk_search_options* opts, uint32_t k_stride, int64_t* out_ids, float* out_scores,
-               uint32_t* out_counts, uint32_t* out_passing, frk_search_stats* stats) {
-    if (!t || !vector_field || !posting_field || (!queries && n_queries) || !ks || !out_ids || !out_counts)
-        return Fail(FRK_ERR_INVALID, "frk_search: null argument");
-    std::vector<std::unique_ptr<QueryNode>> trees(n_queries);
+namespace {
+// AddQuery x n + BatchSearch over prebuilt trees; results are decoded straight from the sorted keys
+// (score from the ordered high word, global id through the id mapping), as BatchSearchExtract does.
+int SearchTrees(const frk_table* t, const char* vector_field, uint32_t n_queries, const float* queries, uint32_t dim,
+                QueryNode* const* trees, const uint32_t* ks, const frk_search_options* opts, uint32_t k_stride,
+                int64_t* out_ids, float* out_scores, uint32_t* out_counts, uint32_t* out_passing,
+                frk_search_stats* stats, std::chrono::steady_clock::time_point tCall) {
     FullRecallSearcher searcher(t->table->GetDeviceId(), t->table);
     for (uint32_t q = 0; q < n_queries; q++) {
         if (ks[q] > k_stride) return Fail(FRK_ERR_CAPACITY, "frk_search: k larger than k_stride");
-        bool ok = true;
-        std::unique_ptr<FilterAst> ast;
-        if (filters && filters[q]) {
-            ast = ParseFilter(filters[q], ok);
-            if (!ok) {
-                frk::SetError("filter %u does not parse: %s", q, filters[q]);
-                return FRK_ERR_INVALID;
-            }
-        }
-        trees[q] = BuildQueryNode(ast.get(), posting_field);
-        searcher.AddQuery(trees[q].get(), queries + static_cast<size_t>(q) * dim, dim, ks[q]);
+        searcher.AddQuery(trees[q], queries + static_cast<size_t>(q) * dim, dim, ks[q]);
     }
     if (opts) {
         SearchOptions o;
@@ -267,15 +263,19 @@ int frk_search(const frk_table* t, const char* vector_field, const char* posting
     }
     std::vector<AggrDeviceResult> dev;
     if (!searcher.BatchSearchDevice(vector_field, false, dev)) return FRK_ERR_INVALID;
-    std::vector<std::shared_ptr<FullRecallResult>> res;
-    if (!searcher.BatchSearchExtract(dev, res, false)) return FRK_ERR_INVALID;
+    const DocIdMapping* map = t->table->GetDocIdMapping();
     for (uint32_t q = 0; q < n_queries; q++) {
-        const auto& r = *res[q];
-        out_counts[q] = static_cast<uint32_t>(r.docIds.size());
-        for (size_t i = 0; i < r.docIds.size(); i++) {
-            out_ids[static_cast<size_t>(q) * k_stride + i] = r.docIds[i];
-            if (out_scores) out_scores[static_cast<size_t>(q) * k_stride + i] = r.scores[i];
+        const std::vector<uint64_t>& keys = dev[q].keys;
+        int64_t* ids = out_ids + static_cast<size_t>(q) * k_stride;
+        float* sc = out_scores ? out_scores + static_cast<size_t>(q) * k_stride : nullptr;
+        for (size_t i = 0; i < keys.size(); i++) {
+            const uint32_t ldoc = 0xFFFFFFFFu - static_cast<uint32_t>(keys[i]);
+            uint64_t gdoc = ldoc;
+            if (map && !map->GetGDocId(ldoc, gdoc)) return Fail(FRK_ERR_INVALID, "frk_search: doc without a global id");
+            ids[i] = static_cast<int64_t>(gdoc);
+            if (sc) sc[i] = frk::KeyScore(keys[i]);
         }
+        out_counts[q] = static_cast<uint32_t>(keys.size());
         if (out_passing) out_passing[q] = dev[q].passing;
     }
     if (stats) {
@@ -291,9 +291,76 @@ int frk_search(const frk_table* t, const char* vector_field, const char* posting
         stats->overflowed = s.overflowed;
         stats->probe_step = s.probeStep;
         stats->graph_replayed = s.graphReplayed ? 1u : 0u;
+        stats->ms_host_prepare = s.ms_host_prepare;
+        stats->ms_host_finish = s.ms_host_finish;
         stats->cand_cap = s.cap;
+        stats->ms_call =
+            std::chrono::duration<float, std::milli>(std::chrono::steady_clock::now() - tCall).count();
     }
     return FRK_OK;
+}
+}  // namespace
+
+int frk_search(const frk_table* t, const char* vector_field, const char* posting_field, uint32_t n_queries,
+               const float* queries, uint32_t dim, const char* const* filters, const uint32_t* ks,
+               const frk_search_options* opts, uint32_t k_stride, int64_t* out_ids, float* out_scores,
+               uint32_t* out_counts, uint32_t* out_passing, frk_search_stats* stats) {
+    const auto tCall = std::chrono::steady_clock::now();
+    if (!t || !vector_field || !posting_field || (!queries && n_queries) || !ks || !out_ids || !out_counts)
+        return Fail(FRK_ERR_INVALID, "frk_search: null argument");
+    std::vector<std::unique_ptr<QueryNode>> trees(n_queries);
+    std::vector<QueryNode*> raw(n_queries);
+    for (uint32_t q = 0; q < n_queries; q++) {
+        bool ok = true;
+        std::unique_ptr<FilterAst> ast;
+        if (filters && filters[q]) {
+            ast = ParseFilter(filters[q], ok);
+            if (!ok) {
+                frk::SetError("filter %u does not parse: %s", q, filters[q]);
+                return FRK_ERR_INVALID;
+            }
+        }
+        trees[q] = BuildQueryNode(ast.get(), posting_field);
+        raw[q] = trees[q].get();
+    }
+    return SearchTrees(t, vector_field, n_queries, queries, dim, raw.data(), ks, opts, k_stride, out_ids, out_scores,
+                       out_counts, out_passing, stats, tCall);
+}
+
+int frk_filter_prepare(const char* posting_field, const char* expr, frk_filter** out) {
+    if (!posting_field || !out) return Fail(FRK_ERR_INVALID, "frk_filter_prepare: null argument");
+    *out = nullptr;
+    bool ok = true;
+    std::unique_ptr<FilterAst> ast;
+    if (expr) {
+        ast = ParseFilter(expr, ok);
+        if (!ok) {
+            frk::SetError("filter does not parse: %s", expr);
+            return FRK_ERR_INVALID;
+        }
+    }
+    auto f = std::make_unique<frk_filter>();
+    f->tree = BuildQueryNode(ast.get(), posting_field);
+    *out = f.release();
+    return FRK_OK;
+}
+
+void frk_filter_free(frk_filter* f) { delete f; }
+
+int frk_search_prepared(const frk_table* t, const char* vector_field, uint32_t n_queries, const float* queries,
+                        uint32_t dim, const frk_filter* const* filters, const uint32_t* ks,
+                        const frk_search_options* opts, uint32_t k_stride, int64_t* out_ids, float* out_scores,
+                        uint32_t* out_counts, uint32_t* out_passing, frk_search_stats* stats) {
+    const auto tCall = std::chrono::steady_clock::now();
+    if (!t || !vector_field || (!queries && n_queries) || !filters || !ks || !out_ids || !out_counts)
+        return Fail(FRK_ERR_INVALID, "frk_search_prepared: null argument");
+    std::vector<QueryNode*> raw(n_queries);
+    for (uint32_t q = 0; q < n_queries; q++) {
+        if (!filters[q]) return Fail(FRK_ERR_INVALID, "frk_search_prepared: null filter handle");
+        raw[q] = filters[q]->tree.get();
+    }
+    return SearchTrees(t, vector_field, n_queries, queries, dim, raw.data(), ks, opts, k_stride, out_ids, out_scores,
+                       out_counts, out_passing, stats, tCall);
 }
 
 int frk_server_create(const frk_table* t, const char* vector_field, const char* posting_field,

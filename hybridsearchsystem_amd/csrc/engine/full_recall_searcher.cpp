@@ -2,6 +2,7 @@
 #include "full_recall_searcher.h"
 
 #include <algorithm>
+#include <chrono>
 #include <cmath>
 #include <limits>
 #include <cmath>
@@ -213,6 +214,11 @@ size_t Pad(size_t n) {
 }  // namespace
 
 bool FullRecallSearcher::RunDevice(const VectorFieldData& vf, std::vector<AggrDeviceResult>& out) {
+    using Clock = std::chrono::steady_clock;
+    const auto tStart = Clock::now();
+    auto msSince = [](Clock::time_point a) {
+        return std::chrono::duration<float, std::milli>(Clock::now() - a).count();
+    };
     const uint32_t Q = static_cast<uint32_t>(m_topKs.size());
     out.assign(Q, AggrDeviceResult());
     m_stats = BatchStats();
@@ -530,7 +536,9 @@ bool FullRecallSearcher::RunDevice(const VectorFieldData& vf, std::vector<AggrDe
             if (const char* what = enqueue(true)) return fail(what);
         }
     }
+    m_stats.ms_host_prepare = msSince(tStart);
     if (hipStreamSynchronize(s) != hipSuccess) return fail("batch synchronize");
+    const auto tSynced = Clock::now();
 
     const uint64_t* hKeys = reinterpret_cast<const uint64_t*>(ho);
     const uint32_t* hCnt = reinterpret_cast<const uint32_t*>(ho + sizeof(uint64_t) * Q * kmax);
@@ -575,6 +583,7 @@ bool FullRecallSearcher::RunDevice(const VectorFieldData& vf, std::vector<AggrDe
         out[q].keys.assign(hKeys + static_cast<size_t>(q) * kmax, hKeys + static_cast<size_t>(q) * kmax + n);
         out[q].passing = m_passing[q];
     }
+    m_stats.ms_host_finish = msSince(tSynced);
     if (redo.empty()) return true;
 
     // ---- rerun only the affected queries: no caller or learned floor (the exact probe floor instead)
@@ -612,6 +621,8 @@ bool FullRecallSearcher::RunDevice(const VectorFieldData& vf, std::vector<AggrDe
     BatchStats merged = first;
     merged.ms_total += m_stats.ms_total;
     merged.candidates += m_stats.candidates;
+    merged.ms_host_prepare += m_stats.ms_host_prepare;
+    merged.ms_host_finish += m_stats.ms_host_finish;
     m_stats = merged;
     m_passing = passing;
     return true;

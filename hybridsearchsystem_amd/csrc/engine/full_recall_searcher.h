@@ -67,6 +67,8 @@ struct BatchStats {
     uint32_t probeStep = 0;
     uint32_t cap = 0;
     bool graphReplayed = false;  // the device sequence ran as a graph replay (stage times then 0)
+    float ms_host_prepare = 0;   // host: from the batch's start to its device sequence enqueued
+    float ms_host_finish = 0;    // host: from the device sync to the per-query results
 };
 
 class SearchWorkspace;

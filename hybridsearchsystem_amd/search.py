@@ -23,6 +23,22 @@ def _mask(dtypes):
     return m
 
 
+class PreparedFilter:
+    """A filter string parsed and built into its query tree once (frk_filter_prepare)."""
+
+    def __init__(self, expr, posting_field="tag"):
+        self.expr = expr
+        h = ctypes.c_void_p()
+        check(lib().frk_filter_prepare(posting_field.encode(), expr.encode() if expr is not None else None,
+                                       ctypes.byref(h)), "frk_filter_prepare")
+        self.h = h
+
+    def __del__(self):
+        if getattr(self, "h", None):
+            lib().frk_filter_free(self.h)
+            self.h = None
+
+
 class Table:
     def __init__(self, handle, vector_field="vec", posting_field="tag", dim=None):
         self.h = handle
@@ -88,6 +104,10 @@ class Table:
             L.frk_builder_free(b)
         return Table(h, vector_field, posting_field, dim)
 
+    def prepare_filters(self, filters):
+        """PreparedFilter handles for a list of filter strings (None = match all)."""
+        return [PreparedFilter(f, self.posting_field) for f in filters]
+
     def floor_features(self, queries, filters, ks):
         """The floor model's 32 features per (query, filter, k), as the searcher computes them."""
         q = np.ascontiguousarray(queries, dtype=np.float32)
@@ -112,6 +132,8 @@ class Table:
     def search(self, queries, filters, ks, dtype=FRK_DTYPE_F32, rescore_factor=4, probe=True, probe_step=0,
                floors=None, cand_cap=0, k_stride=None, floor_model=None, graph=False):
         """Returns (ids int64 [Q, k_stride], scores float32, counts uint32, passing uint32, stats dict).
+        filters: strings (None = match all), or PreparedFilter handles (parsed once, see
+        prepare_filters), or None for no filter at all.
         floor_model: a learned.FloorModel whose floors the searcher predicts on the device.
         graph: replay the device sequence from a hipGraph captured per batch shape."""
         q = np.ascontiguousarray(queries, dtype=np.float32)
@@ -119,7 +141,10 @@ class Table:
         ks = np.ascontiguousarray(np.broadcast_to(np.asarray(ks, dtype=np.uint32), (nq,)))
         k_stride = int(k_stride or max(1, int(ks.max()) if nq else 1))
         fl = None
-        if filters is not None:
+        prepared = filters is not None and len(filters) > 0 and all(isinstance(f, PreparedFilter) for f in filters)
+        if prepared:
+            fl = (ctypes.c_void_p * nq)(*[f.h.value for f in filters])
+        elif filters is not None:
             enc = [f.encode() if f is not None else None for f in filters]
             fl = (ctypes.c_char_p * nq)(*enc)
         opts = SearchOptions(dtype, rescore_factor, int(bool(probe)), probe_step, None, cand_cap, None, int(bool(graph)))
@@ -135,9 +160,15 @@ class Table:
         cnt = np.zeros(nq, dtype=np.uint32)
         passing = np.zeros(nq, dtype=np.uint32)
         st = SearchStats()
-        check(lib().frk_search(self.h, self.vector_field.encode(), self.posting_field.encode(), nq, q.ctypes.data,
-                               dim, fl, ks.ctypes.data, ctypes.byref(opts), k_stride, ids.ctypes.data, sc.ctypes.data,
-                               cnt.ctypes.data, passing.ctypes.data, ctypes.byref(st)), "frk_search")
+        if prepared:
+            check(lib().frk_search_prepared(self.h, self.vector_field.encode(), nq, q.ctypes.data, dim,
+                                            ctypes.cast(fl, ctypes.c_void_p), ks.ctypes.data, ctypes.byref(opts),
+                                            k_stride, ids.ctypes.data, sc.ctypes.data, cnt.ctypes.data,
+                                            passing.ctypes.data, ctypes.byref(st)), "frk_search_prepared")
+        else:
+            check(lib().frk_search(self.h, self.vector_field.encode(), self.posting_field.encode(), nq, q.ctypes.data,
+                                   dim, fl, ks.ctypes.data, ctypes.byref(opts), k_stride, ids.ctypes.data,
+                                   sc.ctypes.data, cnt.ctypes.data, passing.ctypes.data, ctypes.byref(st)), "frk_search")
         stats = {f: getattr(st, f) for f, _ in SearchStats._fields_}
         return ids, sc, cnt, passing, stats
 

@@ -348,6 +348,9 @@ typedef struct {
     float ms_filter, ms_probe, ms_score, ms_select, ms_rescore, ms_total; /* stage times 0 on a replay */
     uint64_t candidates;
     uint32_t reruns, overflowed, probe_step, cand_cap, graph_replayed;
+    /* host wall times: the searcher's preparation up to the enqueue, its result extraction after the
+     * sync, and the whole call (filter parsing and output copies included) */
+    float ms_host_prepare, ms_host_finish, ms_call;
 } frk_search_stats;
 
 /* The FRK_FLOOR_FEATURES floor-model features of n (filter, k) pairs against the table's posting
@@ -364,6 +367,18 @@ int frk_search(const frk_table* t, const char* vector_field, const char* posting
                const float* queries, uint32_t dim, const char* const* filters, const uint32_t* ks,
                const frk_search_options* opts, uint32_t k_stride, int64_t* out_ids, float* out_scores,
                uint32_t* out_counts, uint32_t* out_passing, frk_search_stats* stats);
+
+/* Prepared filters: a filter string parsed and built into its QueryNode tree once (the harness builds
+ * its trees when it loads the query file, harness/query/filter_setup.cpp:129-136), for any number of
+ * later searches from any thread. expr NULL or empty = match all. */
+typedef struct frk_filter frk_filter;
+int frk_filter_prepare(const char* posting_field, const char* expr, frk_filter** out);
+void frk_filter_free(frk_filter* f);
+/* frk_search with prepared filters (one handle per query, shared handles allowed). */
+int frk_search_prepared(const frk_table* t, const char* vector_field, uint32_t n_queries, const float* queries,
+                        uint32_t dim, const frk_filter* const* filters, const uint32_t* ks,
+                        const frk_search_options* opts, uint32_t k_stride, int64_t* out_ids, float* out_scores,
+                        uint32_t* out_counts, uint32_t* out_passing, frk_search_stats* stats);
 
 /* ------------------------------------------------------------------------------------------------
  * Serving: a continuous-batching request server over the searcher (extends the search driver's

@@ -130,3 +130,13 @@ def test_fp8_scales_are_powers_of_two(L):
     nz = mx > 0
     assert np.all(mx[nz] / scales[nz] <= 448.0) and np.all(mx[nz] / scales[nz] > 224.0)
     assert scales[3] == 1.0 and scales[7] == 1.0 and scales[8] == 2.0
+
+
+def test_prepared_filters_parse_once_and_reject_bad_text(L):
+    """Prepared filter handles: built without a device, match-all for None/empty, errors for bad text."""
+    from hybridsearchsystem_amd import FrkError, search
+    handles = [search.PreparedFilter(e) for e in ("(3 AND NOT 5) OR 7", None, "", "12")]
+    assert all(h.h.value for h in handles)
+    for bad in ("(3 AND", "3 OR OR 4", "AND"):
+        with pytest.raises(FrkError, match="does not parse"):
+            search.PreparedFilter(bad)

@@ -192,3 +192,22 @@ def test_any_dimension(search_mod, dim):
         hits = sum(len(set(ids32[q, :cnt32[q]].tolist()) & set(lo[0][q, :lo[2][q]].tolist())) for q in range(nq))
         assert hits / max(1, int(cnt32.sum())) >= min_recall, (dtype, hits, int(cnt32.sum()))
     t.free()
+
+
+def test_prepared_filters_match_filter_strings(search_mod):
+    """Searching with prepared filter handles returns what the same filter strings return."""
+    n, dim, n_tags = 20000, 64, 30
+    docs, dens, off, ids, words = _corpus(91, n, dim, n_tags)
+    qs = synth.mixture_queries(92, 40, dim, 32, corpus_seed=91)
+    filters, _ = synth.random_filters(93, dens, 38, depth=3, sel=(0.001, 0.8))
+    filters = [None, ""] + filters
+    ks = np.random.default_rng(94).choice([1, 10, 100], size=40).astype(np.uint32)
+    t = search_mod.Table.build(docs, off, ids, docs_per_segment=8192, dtypes=(2,))
+    pf = t.prepare_filters(filters)
+    for kw in (dict(), dict(dtype=2, rescore_factor=4, graph=True)):
+        for _ in range(3):  # graph replay from the second batch of a shape on
+            a = t.search(qs, filters, ks, **kw)
+            b = t.search(qs, pf, ks, **kw)
+            for x, y in zip(a[:4], b[:4]):
+                assert np.array_equal(x, y)
+    t.free()
