@@ -17,6 +17,7 @@ namespace frk {
 namespace {
 
 constexpr int kThreads = 1024;
+constexpr int kWaves = kThreads / 64;
 constexpr uint32_t kLdsKeys = 8192;     // candidate keys a workgroup keeps in LDS for the radix passes
 constexpr uint32_t kLdsVals = 16384;    // values the k-th largest kernel keeps in LDS
 
@@ -75,7 +76,9 @@ __global__ __launch_bounds__(kThreads) void SelectKernel(const uint64_t* __restr
                                                          uint64_t* __restrict__ out, uint32_t k_stride,
                                                          uint32_t* __restrict__ out_counts, float* __restrict__ kth) {
     __shared__ uint32_t hist[256];
+    __shared__ uint32_t whist[256 * kWaves];  // per-wave sub-histograms
     __shared__ uint32_t s_digit, s_above, s_fill;
+    __shared__ uint64_t s_min, s_max;
     __shared__ uint64_t sorted[FRK_SELECT_MAX_K];
     __shared__ uint64_t keys[kLdsKeys];
 
@@ -106,44 +109,97 @@ __global__ __launch_bounds__(kThreads) void SelectKernel(const uint64_t* __restr
         }
         return;
     }
-    // MSB-first radix select of the k-th largest key (8 passes of 8-bit digits; keys are unique), over
-    // an LDS copy of the keys when they fit
-    const bool in_lds = n <= kLdsKeys;
-    if (in_lds)
-        for (uint32_t i = threadIdx.x; i < n; i += kThreads) keys[i] = c[i];
-    if (threadIdx.x == 0) s_fill = 0;
+    // MSB-first radix select of the k-th largest key (8-bit digits; keys are unique). The leading
+    // bytes all candidates share are skipped (one min/max pass), every wave counts into its own
+    // sub-histogram (the top digits crowd into few bins), and as soon as the keys still in the running
+    // fit in LDS they are compacted there, so later passes read LDS instead of HBM.
+    const int wave = threadIdx.x >> 6;
+    if (threadIdx.x == 0) {
+        s_fill = 0;
+        s_min = ~0ull;
+        s_max = 0ull;
+    }
+    __syncthreads();
+    {
+        uint64_t lo = ~0ull, hi = 0ull;
+        for (uint32_t i = threadIdx.x; i < n; i += kThreads) {
+            const uint64_t key = c[i];
+            lo = key < lo ? key : lo;
+            hi = key > hi ? key : hi;
+        }
+        for (int off = 32; off > 0; off >>= 1) {
+            const uint64_t l2 = __shfl_xor(lo, off, 64), h2 = __shfl_xor(hi, off, 64);
+            lo = l2 < lo ? l2 : lo;
+            hi = h2 > hi ? h2 : hi;
+        }
+        if ((threadIdx.x & 63) == 0) {
+            atomicMin(reinterpret_cast<unsigned long long*>(&s_min), static_cast<unsigned long long>(lo));
+            atomicMax(reinterpret_cast<unsigned long long*>(&s_max), static_cast<unsigned long long>(hi));
+        }
+    }
+    __syncthreads();
+    const uint64_t kmin = s_min, kmax = s_max;
     uint64_t prefix = 0, pmask = 0;
+    int shift = 56;
+    while (shift > 0 && ((kmin ^ kmax) >> shift) == 0) {  // a byte every key shares is the answer's
+        pmask |= 0xFFull << shift;
+        shift -= 8;
+    }
+    prefix = kmin & pmask;
     uint32_t rem = k;
-    for (int shift = 56; shift >= 0; shift -= 8) {
-        for (int b = threadIdx.x; b < 256; b += kThreads) hist[b] = 0;
+    uint32_t live = n;        // keys matching the prefix
+    uint32_t stored = 0;      // keys compacted into LDS (a superset of the live ones), once in_lds
+    bool in_lds = false;
+    for (; shift >= 0; shift -= 8) {
+        for (int b = threadIdx.x; b < 256 * kWaves; b += kThreads) whist[b] = 0;
         __syncthreads();
-        // defaults, never used while rem <= the count (set after every thread read the last verdict)
         if (threadIdx.x == 0) s_digit = s_above = 0;
-        auto pass = [&](const uint64_t* src) __attribute__((always_inline)) {
-            for (uint32_t i = threadIdx.x; i < n; i += kThreads) {
+        auto pass = [&](const uint64_t* src, uint32_t cnt) __attribute__((always_inline)) {
+            for (uint32_t i = threadIdx.x; i < cnt; i += kThreads) {
                 const uint64_t key = src[i];
-                if ((key & pmask) == prefix) atomicAdd(&hist[(key >> shift) & 255u], 1u);
+                if ((key & pmask) == prefix) atomicAdd(&whist[wave * 256 + ((key >> shift) & 255u)], 1u);
             }
         };
-        if (in_lds) pass(keys); else pass(c);
+        if (in_lds) pass(keys, stored); else pass(c, n);
+        __syncthreads();
+        for (int b = threadIdx.x; b < 256; b += kThreads) {
+            uint32_t t = 0;
+#pragma unroll
+            for (int w = 0; w < kWaves; w++) t += whist[w * 256 + b];
+            hist[b] = t;
+        }
         __syncthreads();
         if (threadIdx.x < 64) FindDigit(hist, rem, &s_digit, &s_above);
         __syncthreads();
-        prefix |= static_cast<uint64_t>(s_digit) << shift;
+        const uint32_t d = s_digit;
+        prefix |= static_cast<uint64_t>(d) << shift;
         rem -= s_above;
         pmask |= 0xFFull << shift;
-    }
-    const uint64_t T = prefix;  // the k-th largest key
-    auto gather = [&](const uint64_t* src) __attribute__((always_inline)) {
-        for (uint32_t i = threadIdx.x; i < n; i += kThreads) {
-            const uint64_t key = src[i];
-            if (key >= T) {
-                const uint32_t at = atomicAdd(&s_fill, 1u);
-                if (at < FRK_SELECT_MAX_K) sorted[at] = key;
+        live = hist[d];
+        if (!in_lds && shift > 0 && live <= kLdsKeys) {
+            // compact the keys still in the running into LDS
+            if (threadIdx.x == 0) s_fill = 0;
+            __syncthreads();
+            for (uint32_t i = threadIdx.x; i < n; i += kThreads) {
+                const uint64_t key = c[i];
+                if ((key & pmask) == prefix) keys[atomicAdd(&s_fill, 1u)] = key;
             }
+            __syncthreads();
+            in_lds = true;
+            stored = live;
         }
-    };
-    if (in_lds) gather(keys); else gather(c);
+    }
+    if (threadIdx.x == 0) s_fill = 0;
+    __syncthreads();
+    const uint64_t T = prefix;  // the k-th largest key
+    // the k keys >= T: those of bins above the chosen digits were never compacted, so read them all
+    for (uint32_t i = threadIdx.x; i < n; i += kThreads) {
+        const uint64_t key = c[i];
+        if (key >= T) {
+            const uint32_t at = atomicAdd(&s_fill, 1u);
+            if (at < FRK_SELECT_MAX_K) sorted[at] = key;
+        }
+    }
     __syncthreads();
     uint32_t p2 = 1;
     while (p2 < k) p2 <<= 1;
@@ -163,8 +219,10 @@ __global__ __launch_bounds__(kThreads) void KthLargestKernel(const float* __rest
                                                              uint32_t n, const uint32_t* __restrict__ ks,
                                                              uint32_t k_default, float* __restrict__ out) {
     __shared__ uint32_t hist[256];
+    __shared__ uint32_t whist[256 * kWaves];  // per-wave sub-histograms
     __shared__ uint32_t s_digit, s_above;
     __shared__ uint32_t okeys[kLdsVals];
+    const int wave = threadIdx.x >> 6;
     const uint32_t q = blockIdx.x;
     const uint32_t k = ks ? ks[q] : k_default;
     const float* v = vals + static_cast<uint64_t>(q) * stride;
@@ -177,12 +235,19 @@ __global__ __launch_bounds__(kThreads) void KthLargestKernel(const float* __rest
         for (uint32_t i = threadIdx.x; i < n; i += kThreads) okeys[i] = OrderedFromFloat(v[i]);
     uint32_t prefix = 0, mask = 0, rem = k;
     for (int shift = 24; shift >= 0; shift -= 8) {
-        for (int b = threadIdx.x; b < 256; b += kThreads) hist[b] = 0;
+        for (int b = threadIdx.x; b < 256 * kWaves; b += kThreads) whist[b] = 0;
         __syncthreads();
         if (threadIdx.x == 0) s_digit = s_above = 0;
         for (uint32_t i = threadIdx.x; i < n; i += kThreads) {
             const uint32_t key = in_lds ? okeys[i] : OrderedFromFloat(v[i]);
-            if ((key & mask) == prefix) atomicAdd(&hist[(key >> shift) & 255u], 1u);
+            if ((key & mask) == prefix) atomicAdd(&whist[wave * 256 + ((key >> shift) & 255u)], 1u);
+        }
+        __syncthreads();
+        for (int b = threadIdx.x; b < 256; b += kThreads) {
+            uint32_t t = 0;
+#pragma unroll
+            for (int w = 0; w < kWaves; w++) t += whist[w * 256 + b];
+            hist[b] = t;
         }
         __syncthreads();
         if (threadIdx.x < 64) FindDigit(hist, rem, &s_digit, &s_above);

@@ -334,16 +334,19 @@ def test_rescore_from_tiles_and_rows(frk):
 
 def test_select_and_kth_largest_at_every_path_boundary(frk):
     """Top-k select against numpy on unique keys at candidate counts around the paths: sorted outright
-    (<= 4096), radix passes over an LDS copy (<= 8192), radix passes over HBM; k above and below n,
-    counts above the capacity. The k-th largest float likewise around its LDS size (16384)."""
+    (<= 4096), radix passes over an LDS copy (<= 8192), radix passes over HBM until the keys still in
+    the running fit LDS (120,000 keys in a narrow score range: shared leading bytes skipped); k above
+    and below n, counts above the capacity. The k-th largest float likewise around its LDS size (16384)."""
     from hybridsearchsystem_amd import check, lib
     rng = np.random.default_rng(81)
-    sizes = [0, 1, 100, 4095, 4096, 4097, 8192, 8193, 20000]
+    sizes = [0, 1, 100, 4095, 4096, 4097, 8192, 8193, 20000, 120000]
     cap = max(sizes)
     nq = len(sizes)
     keys = np.zeros((nq, cap), np.uint64)
     for q, n in enumerate(sizes):
         sc = rng.standard_normal(n).astype(np.float32)
+        if n == 120000:
+            sc = (1.0 + 1e-4 * sc).astype(np.float32)  # one top byte and most second bytes shared
         sc[: n // 7] = 0.5  # ties in score, broken by doc id
         docs = rng.choice(1 << 31, n, replace=False).astype(np.uint64)
         u = sc.view(np.uint32).astype(np.uint64)
