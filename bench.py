@@ -53,18 +53,35 @@ def build_table(n_docs, dim, n_tags, centroids, dtypes, seed, keep_rows, chunk=1
     return table, dens, off, ids, head
 
 
-def run_steps(table, queries, filters, ks, steps, batch, **kw):
-    """Times `steps` batches; returns (per-batch ms list, last stats, results of every batch)."""
+def run_steps(table, queries, filters, ks, steps, batch, streams=1, **kw):
+    """Runs `steps` batches -- one after another, or spread round-robin over `streams` host threads,
+    each searching on its own workspace and HIP stream so that batches overlap on the device (the
+    search driver's multi-threaded mode). Returns (per-batch ms list, per-batch stats, results of
+    every batch), in step order."""
     n = queries.shape[0]
-    lat, results, stats = [], [], []
-    for s in range(steps):
+    lat, results, stats = [None] * steps, [None] * steps, [None] * steps
+
+    def one(s):
         lo = (s * batch) % n
         sl = slice(lo, lo + batch)
         t0 = time.perf_counter()
         r = table.search(queries[sl], filters[sl], ks[sl], **kw)
-        lat.append((time.perf_counter() - t0) * 1e3)
-        results.append((lo, r[0], r[2]))
-        stats.append(r[4])
+        lat[s] = (time.perf_counter() - t0) * 1e3
+        results[s] = (lo, r[0], r[2])
+        stats[s] = r[4]
+
+    if streams <= 1:
+        for s in range(steps):
+            one(s)
+    else:
+        def worker(t):
+            for s in range(t, steps, streams):
+                one(s)
+        th = [threading.Thread(target=worker, args=(t,)) for t in range(streams)]
+        for t in th:
+            t.start()
+        for t in th:
+            t.join()
     return lat, stats, results
 
 
@@ -152,21 +169,46 @@ DTYPE_NAMES = {0: "f32", 1: "bf16", 2: "fp8"}
 ELEM_BYTES = {0: 4, 1: 2, 2: 1}
 
 
-def timed_leg(table, queries, filters, ks, args, res_p, name, dtype, **kw):
+def timed_leg(table, queries, filters, ks, args, res_p, name, dtype, streams=1, **kw):
     """Warms up, then times args.steps batches (device sequence replayed from a hipGraph per batch
-    shape unless --eager); returns the run tuple the chooser compares."""
+    shape unless --eager; `streams` batches in flight); returns the run tuple the chooser compares."""
     import torch
     kw["graph"] = not args.eager
-    run_steps(table, queries, filters, ks, args.warmup, args.batch, dtype=dtype, **kw)
+    run_steps(table, queries, filters, ks, max(args.warmup, streams * 2), args.batch, streams=streams, dtype=dtype,
+              **kw)
     torch.cuda.synchronize()
     t0 = time.perf_counter()
-    lat, st, res = run_steps(table, queries, filters, ks, args.steps, args.batch, dtype=dtype, **kw)
+    lat, st, res = run_steps(table, queries, filters, ks, args.steps, args.batch, streams=streams, dtype=dtype, **kw)
     torch.cuda.synchronize()
     wall = time.perf_counter() - t0
     rec = recall(res, res_p)
     qps = args.steps * args.batch / wall
+    if streams > 1:
+        name += ", %d batch streams" % streams
     log("%s: %.0f q/s, recall %.5f, p99 %.2f ms" % (name, qps, rec, float(np.percentile(lat, 99))))
-    return (name, qps, rec, lat, st, wall, dtype, dict(kw, dtype=dtype))
+    return (name, qps, rec, lat, st, wall, dtype, dict(kw, dtype=dtype), res)
+
+
+def precision_report(table, runs, docs, queries, filters, ks, args):
+    """Per (store, m): recall@k against plain FP32 over the timed stream, and the returned scores' error
+    against FP64 dot products on the host copy of the docs (the first batch, searched once more with
+    the run's settings), relative to sum_d |q_d v_d|."""
+    out = {}
+    b = args.batch
+    q64 = queries[:b].astype(np.float64)
+    for r in runs:
+        ids, sc, cnt, _, _ = table.search(queries[:b], filters[:b], ks[:b], **r[7])
+        err = 0.0
+        for q in range(b):
+            n = int(cnt[q])
+            if n == 0:
+                continue
+            v = docs[ids[q, :n]].astype(np.float64)
+            exact = v @ q64[q]
+            scale = np.abs(v * q64[q]).sum(axis=1)
+            err = max(err, float(np.max(np.abs(sc[q, :n] - exact) / np.maximum(scale, 1e-30))))
+        out[r[0]] = {"recall_at_k_vs_plain": round(r[2], 5), "max_score_err_vs_fp64": float("%.3g" % err)}
+    return out
 
 
 def learned_leg(table, dens, args, queries, filters, ks, res_p, centroids, seed, legs):
@@ -186,13 +228,13 @@ def learned_leg(table, dens, args, queries, filters, ks, res_p, centroids, seed,
     log("learned floor: trained (%s)" % info)
     del tq
     runs, timed = [], {}
-    for dname, dtype, m in legs:
+    for dname, dtype, m, streams in legs:
         name = "%s + learned floor + fp32 rescore of top k*%d" % (dname, m)
-        r = timed_leg(table, queries, filters, ks, args, res_p, name, dtype, rescore_factor=m, probe=False,
-                      floor_model=model)
+        r = timed_leg(table, queries, filters, ks, args, res_p, name, dtype, streams=streams, rescore_factor=m,
+                      probe=False, floor_model=model)
         runs.append(r)
         lat, st = r[3], r[4]
-        timed["%s_rescore%d" % (dname, m)] = {
+        timed["%s_rescore%d%s" % (dname, m, "_x%d" % streams if streams > 1 else "")] = {
             "qps": round(r[1], 2), "p50_ms": round(float(np.percentile(lat, 50)), 3),
             "p99_ms": round(float(np.percentile(lat, 99)), 3), "recall_at_k_vs_plain": round(r[2], 5),
             "candidates_per_query": round(float(np.mean([s["candidates"] for s in st])) / args.batch, 1),
@@ -204,7 +246,8 @@ def learned_leg(table, dens, args, queries, filters, ks, res_p, centroids, seed,
     ef, _ = synth.random_filters(S["eval_filters"], dens, n, depth=3, sel=(1e-5, 0.6))
     ek = synth.draw_ks(S["eval_kdraw"], n, [10, 100, 1000])
     ev = {}
-    for name, dtype, m in [("plain_fp32", FRK_DTYPE_F32, 1)] + [("%s_rescore%d" % (d, m), t, m) for d, t, m in legs]:
+    eval_legs = sorted({(d, t, m) for d, t, m, _ in legs}, key=lambda x: (x[1], x[2]))
+    for name, dtype, m in [("plain_fp32", FRK_DTYPE_F32, 1)] + [("%s_rescore%d" % (d, m), t, m) for d, t, m in eval_legs]:
         cand = {"learned": 0}
         passing = reruns = exact = hits = tot = 0
         for lo in range(0, n, args.batch):
@@ -287,13 +330,20 @@ def main():
     qps_p = args.steps * args.batch / wall_p
     # per-stage HIP-event times of the plain pipeline: an eager pass over the same stream
     _, st_pe, _ = run_steps(table, queries, filters, ks, args.steps, args.batch, dtype=FRK_DTYPE_F32, graph=False)
-    # low-precision legs: (store, rescoring factor m); FP32 rescoring of the top k*m recovers recall
-    legs = [("bf16", FRK_DTYPE_BF16, 4), ("fp8", FRK_DTYPE_FP8E4M3, 4), ("fp8", FRK_DTYPE_FP8E4M3, 8)]
+    # low-precision legs: (store, rescoring factor m, batch streams); FP32 rescoring of the top k*m
+    # recovers recall. m in {2, 4, 8} for both stores with the probe floor (the per-m precision report);
+    # the fast ones also with the learned floor and with two batches in flight.
+    probe_legs = [(d, t, m, 1) for d, t in (("bf16", FRK_DTYPE_BF16), ("fp8", FRK_DTYPE_FP8E4M3)) for m in (2, 4, 8)]
+    probe_legs += [("fp8", FRK_DTYPE_FP8E4M3, 4, 2), ("fp8", FRK_DTYPE_FP8E4M3, 8, 2)]
     runs = [timed_leg(table, queries, filters, ks, args, res_p, "%s + probe floor + fp32 rescore of top k*%d" % (d, m),
-                      t, rescore_factor=m) for d, t, m in legs]
+                      t, streams=n_s, rescore_factor=m) for d, t, m, n_s in probe_legs]
+    # score error vs FP64 needs the host copy of every doc (kept for the CPU baseline unless --no-cpu)
+    precision = precision_report(table, runs[:6], head, queries, filters, ks, args) if head.shape[0] == args.docs else None
     learned_out = None
     if not args.no_learned:
-        learned_out, lruns = learned_leg(table, dens, args, queries, filters, ks, res_p, centroids, seed, legs)
+        learned_legs = [("bf16", FRK_DTYPE_BF16, 4, 1), ("fp8", FRK_DTYPE_FP8E4M3, 4, 1), ("fp8", FRK_DTYPE_FP8E4M3, 8, 1),
+                        ("fp8", FRK_DTYPE_FP8E4M3, 4, 2), ("fp8", FRK_DTYPE_FP8E4M3, 8, 2)]
+        learned_out, lruns = learned_leg(table, dens, args, queries, filters, ks, res_p, centroids, seed, learned_legs)
         runs += lruns
     # the fastest low-precision pipeline whose recall@k against plain FP32 is >= 0.99
     ok_runs = [r for r in runs if r[2] >= 0.99]
@@ -344,6 +394,7 @@ def main():
                        "p99_ms": round(float(np.percentile(lat_p, 99)), 3), "score_kernel_ms": round(ms_score_p, 3),
                        "roofline": {"bound": "mfma", "achieved": round(tf_p, 1), "peak": MFMA_PEAK_TFLOPS["f32"],
                                     "unit": "TFLOP/s", "frac": round(tf_p / MFMA_PEAK_TFLOPS["f32"], 4)}},
+        "precision_by_m": precision,
         "chosen": {"qps": round(value, 2), "pipeline": best[0] if chosen_ok else "plain fp32",
                    "speedup_vs_plain": round(value / qps_p, 3),
                    "candidates": {r[0]: {"qps": round(r[1], 2), "recall": round(r[2], 5),

@@ -35,7 +35,10 @@ class PreparedFilter:
 
     def __del__(self):
         if getattr(self, "h", None):
-            lib().frk_filter_free(self.h)
+            try:
+                lib().frk_filter_free(self.h)
+            except Exception:  # interpreter shutdown: the library may already be gone
+                pass
             self.h = None
 
 
@@ -52,7 +55,10 @@ class Table:
         self.n_docs, self.n_segments, self.docs_per_segment = int(n.value), int(s.value), int(d.value)
 
     def __del__(self):
-        self.free()
+        try:
+            self.free()
+        except Exception:  # interpreter shutdown: the library may already be gone
+            pass
 
     def free(self):
         if getattr(self, "h", None):
