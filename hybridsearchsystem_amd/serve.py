@@ -7,10 +7,10 @@ target rate in queries/s) from one thread while another collects them, and repor
 latency (finish - submit on the server's clock) percentiles, achieved queries/s, batches formed,
 graph replays and the doc-stream rate against the HBM peak.
 
-``python -m hybridsearchsystem_amd.serve`` builds the synthetic serving corpus (20M x 768 BF16 with
-the FP32 originals resident, 4,096 Zipf tags), trains the learned floor, and sweeps arrival rates for
-the chosen pipeline (BF16 + learned floor + FP32 rescoring, graph replay) and the plain FP32 path,
-printing one JSON line per run.
+``python -m hybridsearchsystem_amd.serve`` builds the synthetic serving corpus (20M x 768 with BF16
+and FP8 stores beside the FP32 originals, 4,096 Zipf tags), trains the learned floor, and sweeps
+arrival rates for each low-precision pipeline (store + learned floor + FP32 rescoring, graph replay)
+and the plain FP32 path, printing one JSON line per run.
 """
 import argparse
 import ctypes
@@ -190,6 +190,8 @@ def main():
     ap.add_argument("--seconds", type=float, default=8.0, help="offered load per rate")
     ap.add_argument("--train", type=int, default=50_000, help="learned-floor training queries")
     ap.add_argument("--workers", type=int, default=2)
+    ap.add_argument("--dtypes", default="bf16,fp8", help="low-precision stores to serve from, each with the "
+                    "learned floor and FP32 rescoring")
     ap.add_argument("--plain", action="store_true", help="also run the plain FP32 path at each rate")
     ap.add_argument("--diag", action="store_true", help="compare host- and device-side learned floors first")
     args = ap.parse_args()
@@ -204,11 +206,13 @@ def main():
     for at in range(0, args.docs, 1 << 20):
         m = min(1 << 20, args.docs - at)
         vec[at:at + m] = synth.mixture_vectors_torch(seed, m, args.dim, cents, first=at)
-    table = search.Table.build(vec, off, ids, n_docs=args.docs, dtypes=(search.FRK_DTYPE_BF16,),
+    names = {"bf16": search.FRK_DTYPE_BF16, "fp8": search.FRK_DTYPE_FP8E4M3}
+    dtypes = [d.strip() for d in args.dtypes.split(",") if d.strip()]
+    table = search.Table.build(vec, off, ids, n_docs=args.docs, dtypes=tuple(names[d] for d in dtypes),
                                docs_per_segment=131072)
     del vec
     torch.cuda.empty_cache()
-    log("serving corpus %d x %d built in %.1fs" % (args.docs, args.dim, time.time() - t0))
+    log("serving corpus %d x %d (%s) built in %.1fs" % (args.docs, args.dim, "+".join(dtypes), time.time() - t0))
     S = synth.SEEDS
     tq = synth.mixture_queries(S["train_queries"], args.train, args.dim, cents, corpus_seed=seed)
     tf, _ = synth.random_filters(S["train_filters"], dens, args.train, depth=4, sel=(1e-5, 0.6))
@@ -246,8 +250,8 @@ def main():
                 rr += r[4]["reruns"]
                 ov += r[4]["overflowed"]
             log("diag %s: reruns %d overflowed %d of 1024" % (name, rr, ov))
-    runs = [("bf16 + learned floor + fp32 rescore, graph replay", args.rates,
-             dict(dtype=search.FRK_DTYPE_BF16, rescore_factor=4, probe=False, floor_model=model, graph=True))]
+    runs = [("%s + learned floor + fp32 rescore, graph replay" % d, args.rates,
+             dict(dtype=names[d], rescore_factor=4, probe=False, floor_model=model, graph=True)) for d in dtypes]
     if args.plain:
         runs.append(("plain fp32, graph replay", args.plain_rates,
                      dict(dtype=search.FRK_DTYPE_F32, probe=True, graph=True)))
