@@ -85,8 +85,9 @@ int Prepare(const frk_doc_store* store, const float* d_queries, uint32_t n_queri
     A.cap = cap;
     A.counts = d_counts;
     L.chunks = ch;
-    // at most one workgroup per 4 blocks a wave (the launch lowers this to the resident count)
-    const uint64_t want = (block_count + 4 * kWaves - 1) / (4 * kWaves);
+    // at most one workgroup per 16 blocks (4 per wave of a 4-wave workgroup; the launch lowers this
+    // to the resident count)
+    const uint64_t want = (block_count + 15) / 16;
     L.grid_x = static_cast<uint32_t>(want < 2048 ? (want ? want : 1) : 2048);
     L.stream = static_cast<hipStream_t>(stream);
     return FRK_OK;

@@ -33,12 +33,33 @@ typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
 typedef int intx8 __attribute__((ext_vector_type(8)));
 typedef unsigned int u32x4 __attribute__((ext_vector_type(4)));
 
-constexpr int kThreads = 256;
-constexpr int kWaves = kThreads / 64;
 constexpr uint32_t kXcds = 8;  // accelerator complex dies of an MI355X, each with its own L2
-#ifndef FRK_SCORE_MIN_BLOCKS
-#define FRK_SCORE_MIN_BLOCKS 1  // workgroups per CU the register budget is sized for
+#ifndef FRK_SCORE_STAGE_A
+#define FRK_SCORE_STAGE_A 1
 #endif
+#ifndef FRK_SCORE_WIDE_TILE
+#define FRK_SCORE_WIDE_TILE 64  // query tiles above this many KiB get 8-wave workgroups
+#endif
+
+// Workgroup size of a shape: 4 waves, or 8 where the query tile (RB x CH KiB of LDS) is so large that
+// a CU holds only one workgroup; 8 waves then still give each SIMD two waves, one streaming its doc
+// tiles while the other runs its MFMAs. All waves of a workgroup share its one query tile. Measured on
+// 10M docs: FP8 768 dims 1.56 -> 1.51 ms at 128 queries and 2.30 -> 2.05 ms at 256, FP32 256 dims
+// 5.12 -> 4.99 ms at 128; BF16 is kept at 4 waves, where 8 measured slower (768 dims, 128 queries:
+// 2.83 -> 3.41 ms; 1024 dims: 1.82 -> 2.31 ms on 5M docs).
+template <int DT, int CH, int RB>
+struct WgShape {
+    // (FP8 at a 128 KiB tile would spill under the 256-register cap of two waves per SIMD)
+    static constexpr int kThreads = DT != FRK_DTYPE_BF16 && RB * CH > FRK_SCORE_WIDE_TILE &&
+                                            RB * CH <= (DT == FRK_DTYPE_FP8E4M3 ? 96 : 128)
+                                        ? 512
+                                        : 256;
+    static constexpr int kWaves = kThreads / 64;
+    // Query fragments staged in registers one group ahead: a 4-wave workgroup whose query tile leaves
+    // room for only one workgroup per CU runs a single wave per SIMD, which has no other wave to cover
+    // the latency of an LDS read issued just before its MFMA (and has the whole SIMD's registers).
+    static constexpr bool kStageA = kThreads == 256 && RB * CH > FRK_SCORE_WIDE_TILE && FRK_SCORE_STAGE_A;
+};
 
 template <int DT>
 struct Traits;
@@ -110,6 +131,7 @@ struct ScoreArgs {
 template <int DT, int CH, int RB>
 __device__ __forceinline__ void StageQueries(const ScoreArgs& A, u32x4* a_lds, float* q_scale_lds, uint32_t q0) {
     constexpr int E = Traits<DT>::kElemsPerLane;
+    constexpr int kThreads = WgShape<DT, CH, RB>::kThreads;
     if constexpr (DT == FRK_DTYPE_FP8E4M3) {
         for (int r = threadIdx.x; r < 32 * RB; r += kThreads) {
             uint32_t q = q0 + r;
@@ -183,26 +205,22 @@ struct GroupShape {
 };
 
 // qexp: per row block the E8M0 exponent of this lane's query row scale; dexp: the doc's (FP8 only).
-template <int DT, int CH, int RB>
-__device__ __forceinline__ void ComputeGroup(const u32x4* a_lds, int g, const u32x4 (&buf)[GroupShape<CH>::G],
-                                    floatx16 (&acc)[RB], const int (&qexp)[RB], int dexp) {
+// a(rb, c): this lane's query fragment of row block rb and chunk c of the group.
+template <int DT, int CH, int RB, typename FragA>
+__device__ __forceinline__ void ComputeGroup(FragA&& a, const u32x4 (&buf)[GroupShape<CH>::G], floatx16 (&acc)[RB],
+                                    const int (&qexp)[RB], int dexp) {
     constexpr int G = GroupShape<CH>::G;
-    const int lane = threadIdx.x & 63;
     if constexpr (DT == FRK_DTYPE_FP8E4M3) {
 #pragma unroll
         for (int c = 0; c < G; c += 2) {
 #pragma unroll
-            for (int rb = 0; rb < RB; rb++) {
-                const u32x4 a0 = a_lds[(rb * CH + g * G + c) * 64 + lane];
-                const u32x4 a1 = a_lds[(rb * CH + g * G + c + 1) * 64 + lane];
-                MfmaChunkPairFp8(a0, a1, buf[c], buf[c + 1], acc[rb], qexp[rb], dexp);
-            }
+            for (int rb = 0; rb < RB; rb++) MfmaChunkPairFp8(a(rb, c), a(rb, c + 1), buf[c], buf[c + 1], acc[rb], qexp[rb], dexp);
         }
     } else {
 #pragma unroll
         for (int c = 0; c < G; c++) {
 #pragma unroll
-            for (int rb = 0; rb < RB; rb++) MfmaChunk<DT>(a_lds[(rb * CH + g * G + c) * 64 + lane], buf[c], acc[rb]);
+            for (int rb = 0; rb < RB; rb++) MfmaChunk<DT>(a(rb, c), buf[c], acc[rb]);
         }
     }
 }
@@ -443,7 +461,9 @@ __device__ __forceinline__ void EpilogueBlockMax(const ScoreArgs& A, uint32_t q0
 }
 
 template <int DT, int CH, int RB, bool kBlockMax>
-__global__ __launch_bounds__(kThreads, FRK_SCORE_MIN_BLOCKS) void ScoreKernel(ScoreArgs A) {
+__global__ __launch_bounds__((WgShape<DT, CH, RB>::kThreads), 1) void ScoreKernel(ScoreArgs A) {
+    constexpr int kThreads = WgShape<DT, CH, RB>::kThreads;
+    constexpr int kWaves = WgShape<DT, CH, RB>::kWaves;
     constexpr int G = GroupShape<CH>::G;
     constexpr int NG = GroupShape<CH>::kGroups;
     extern __shared__ u32x4 smem[];
@@ -488,6 +508,7 @@ __global__ __launch_bounds__(kThreads, FRK_SCORE_MIN_BLOCKS) void ScoreKernel(Sc
     const uint64_t n_mine = (i_hi - i_lo - wave + kWaves - 1) / kWaves;  // blocks of this wave
     const uint64_t total = n_mine * NG;                                   // (block, group) steps
     constexpr int U = GroupShape<CH>::kUnroll;
+    static_assert(U % 2 == 0 || !WgShape<DT, CH, RB>::kStageA, "staged fragment buffers alternate by step");
 
     auto block_index = [&](uint64_t step) -> uint64_t { return i_lo + wave + (step / NG) * kWaves; };
     u32x4 ring[kRing][G];
@@ -500,6 +521,18 @@ __global__ __launch_bounds__(kThreads, FRK_SCORE_MIN_BLOCKS) void ScoreKernel(Sc
             qexp[rb] = static_cast<int>((__float_as_uint(q_scale_lds[32 * rb + (lane & 31)]) >> 23) & 0xFFu);
     }
     int dexp = 127;
+    // staged query fragments (kStageA): two buffers, the step's group and the next step's
+    constexpr bool kStageA = WgShape<DT, CH, RB>::kStageA;
+    u32x4 afrag[kStageA ? 2 : 1][G][RB];
+    auto load_a = [&](auto buf_c, auto group_c) __attribute__((always_inline)) {
+        constexpr int B = decltype(buf_c)::value;
+        constexpr int GL = decltype(group_c)::value;
+#pragma unroll
+        for (int c = 0; c < G; c++)
+#pragma unroll
+            for (int rb = 0; rb < RB; rb++) afrag[B][c][rb] = a_lds[(rb * CH + GL * G + c) * 64 + lane];
+    };
+    if constexpr (kStageA) load_a(std::integral_constant<int, 0>{}, std::integral_constant<int, 0>{});
     // load step `step` (group index GL, a compile-time constant) into ring slot SL
     auto load = [&](auto slot_c, auto group_c, uint64_t step) __attribute__((always_inline)) {
         constexpr int SL = decltype(slot_c)::value;
@@ -534,7 +567,19 @@ __global__ __launch_bounds__(kThreads, FRK_SCORE_MIN_BLOCKS) void ScoreKernel(Sc
             InitBias<RB>(mring[S], blk * 32 + (lane & 31) < A.n_docs, acc);
             if constexpr (DT == FRK_DTYPE_FP8E4M3) dexp = static_cast<int>((__float_as_uint(sring[S]) >> 23) & 0xFFu);
         }
-        ComputeGroup<DT, CH, RB>(a_lds, g, ring[S], acc, qexp, dexp);
+        if constexpr (kStageA) {
+            // the next step's fragments are read before this step's MFMAs are issued, so their LDS
+            // latency runs under those MFMAs; U is a multiple of kRing, hence even, so buffer s % 2
+            // continues across loop bodies
+            load_a(std::integral_constant<int, (s + 1) % 2>{}, std::integral_constant<int, (s + 1) % NG>{});
+            __builtin_amdgcn_sched_barrier(0);
+            ComputeGroup<DT, CH, RB>([&](int rb, int c) -> const u32x4& { return afrag[s % 2][c][rb]; }, ring[S], acc,
+                                     qexp, dexp);
+        } else {
+            ComputeGroup<DT, CH, RB>(
+                [&](int rb, int c) -> u32x4 { return a_lds[(rb * CH + g * G + c) * 64 + lane]; }, ring[S], acc,
+                qexp, dexp);
+        }
         if constexpr (g == NG - 1) {
             // every row block's MFMAs are issued before the first epilogue; left alone, the compiler
             // sinks a row block's MFMAs past the epilogue branches of the row blocks before it, and the
@@ -569,6 +614,8 @@ constexpr uint32_t kScoreShapes[] = {4, 8, 12, 16, 24, 32, 48, 64, 96, 128};
 
 template <int DT, int CH, int RB, bool kBlockMax>
 int Launch(const ScoreArgs& A, hipStream_t stream, uint32_t grid_x) {
+    constexpr int kThreads = WgShape<DT, CH, RB>::kThreads;
+    constexpr int kWaves = WgShape<DT, CH, RB>::kWaves;
     const size_t lds = static_cast<size_t>(RB) * CH * 64 * sizeof(u32x4) + 2 * 32 * RB * sizeof(float) +
                        kWaves * (kQueueKeys + kQueueRuns) * sizeof(uint64_t);
     const void* fn = reinterpret_cast<const void*>(&ScoreKernel<DT, CH, RB, kBlockMax>);
@@ -605,8 +652,13 @@ int Launch(const ScoreArgs& A, hipStream_t stream, uint32_t grid_x) {
 #endif
 
 // Instantiated shapes: CH = 16-byte chunks per doc (FP32 dims 32..1024, BF16 64..2048, FP8 128..4096
-// at the listed counts; kScoreShapes lists them for frk_store_dim, which pads a store's dims up to one). RB = 32-query row blocks per workgroup, at most what keeps the query tile
-// (RB * CH KiB) within 128 KiB of the 160 KiB LDS.
+// at the listed counts; kScoreShapes lists them for frk_store_dim, which pads a store's dims up to one).
+// RB = 32-query row blocks per workgroup, at most what keeps the query tile (RB * CH KiB) within
+// 128 KiB of the 160 KiB LDS; 48 chunks (BF16 768 dims, FP32 384, FP8 1536) also take 3 row blocks (144 KiB, with the
+// 4-wave queue 155 KiB), used for the main pass where it needs fewer query groups than 2 row blocks:
+// every group re-reads the doc range (from the XCD's L2 after the first). Measured on 10M x 768 BF16
+// (with the staged query fragments): 192 queries 4.02 -> 3.53 ms, 128 queries (still two groups of 64)
+// 2.91 -> 2.75 ms from the staging alone.
 template <int DT, bool kBlockMax>
 int Dispatch(const ScoreArgs& A, uint32_t ch, hipStream_t stream, uint32_t grid_x) {
     const uint32_t q = A.n_queries;
@@ -614,10 +666,14 @@ int Dispatch(const ScoreArgs& A, uint32_t ch, hipStream_t stream, uint32_t grid_
     if (rb > FRK_SCORE_MAX_RB) rb = FRK_SCORE_MAX_RB;
     // FP8 probe pass: two query groups of two row blocks (sharing an XCD's L2) measured faster
     if (DT == FRK_DTYPE_FP8E4M3 && kBlockMax && rb > FRK_SCORE_FP8_PROBE_MAX_RB) rb = FRK_SCORE_FP8_PROBE_MAX_RB;
+    const bool three = rb == 4 && (q + 95) / 96 < (q + 63) / 64;
 #define FRK_CASE(CHV)                                                                           \
     if (ch == CHV) {                                                                            \
         if constexpr (CHV * 4 <= 128) {                                                         \
             if (rb == 4) return Launch<DT, CHV, 4, kBlockMax>(A, stream, grid_x);               \
+        }                                                                                       \
+        if constexpr (CHV == 48 && !kBlockMax) {                                                \
+            if (three) return Launch<DT, CHV, 3, kBlockMax>(A, stream, grid_x);                 \
         }                                                                                       \
         if constexpr (CHV * 2 <= 128) {                                                         \
             if (rb >= 2) return Launch<DT, CHV, 2, kBlockMax>(A, stream, grid_x);               \
