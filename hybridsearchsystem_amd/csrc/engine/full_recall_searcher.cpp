@@ -385,9 +385,12 @@ bool FullRecallSearcher::RunDevice(const VectorFieldData& vf, std::vector<AggrDe
     const uint64_t probeBlocks = useProbe ? (nBlocks + probeStep - 1) / probeStep : 0;
     m_stats.probeStep = useProbe ? probeStep : 0;
 
-    // counters: [filter counts Q][cand counts Q][status 64][out counts Q][lp counts Q]
-    const size_t cFilter = 0, cCand = Q, cStatus = 2 * Q, cOut = 2 * Q + 64, cLp = 3 * Q + 64;
-    const size_t nCounters = 4 * Q + 64;
+    // counters: [filter counts Q][status 64][out counts Q][lp counts Q][cand counts Q x kCandStride]; the
+    // scorer's per-query candidate counters sit one 128-byte line apart, so their atomics spread over
+    // the L2 channels
+    constexpr uint32_t kCandStride = 32;
+    const size_t cFilter = 0, cStatus = Q, cOut = Q + 64, cLp = 2 * Q + 64, cCand = 3 * Q + 64;
+    const size_t nCounters = 3 * Q + 64 + static_cast<size_t>(kCandStride) * Q;
     float* dKth = nullptr;
     const uint32_t qStride = (Q + 31) / 32 * 32;  // per-doc filter words: qStride u32 per doc block
     if (!ws->bitmaps.Ensure(sizeof(uint64_t) * Q * maskStride) || !ws->bmasks.Ensure(sizeof(uint32_t) * qStride * nBlocks) || !ws->counters.Ensure(sizeof(uint32_t) * nCounters + 4 * Q) ||
@@ -454,17 +457,17 @@ bool FullRecallSearcher::RunDevice(const VectorFieldData& vf, std::vector<AggrDe
         }
         rec(2);
         if (frk_score_select(store, dQMain, Q, dBMask, qStride, dFloor, 0, 1, nBlocks, ws->cand.As<uint64_t>(), cap,
-                             dCnt + cCand, s) != 0)
+                             dCnt + cCand, kCandStride, s) != 0)
             return "score";
         rec(3);
         if (!lowp) {
-            if (frk_select_topk(ws->cand.As<uint64_t>(), cap, dCnt + cCand, Q, dK, 0, ws->out.As<uint64_t>(), kmax,
+            if (frk_select_topk(ws->cand.As<uint64_t>(), cap, dCnt + cCand, kCandStride, Q, dK, 0, ws->out.As<uint64_t>(), kmax,
                                 dCnt + cOut, nullptr, s) != 0)
                 return "select";
             rec(4);
             rec(5);
         } else {
-            if (frk_select_topk(ws->cand.As<uint64_t>(), cap, dCnt + cCand, Q, dKm, 0, ws->lp.As<uint64_t>(), kSel,
+            if (frk_select_topk(ws->cand.As<uint64_t>(), cap, dCnt + cCand, kCandStride, Q, dKm, 0, ws->lp.As<uint64_t>(), kSel,
                                 dCnt + cLp, nullptr, s) != 0)
                 return "select k*m";
             rec(4);
@@ -474,7 +477,7 @@ bool FullRecallSearcher::RunDevice(const VectorFieldData& vf, std::vector<AggrDe
                                    : frk_rescore(f32, dQF32, Q, ws->lp.As<uint64_t>(), kSel, dCnt + cLp,
                                                  ws->rescored.As<uint64_t>(), s);
             if (rc != 0) return "rescore";
-            if (frk_select_topk(ws->rescored.As<uint64_t>(), kSel, dCnt + cLp, Q, dK, 0, ws->out.As<uint64_t>(), kmax,
+            if (frk_select_topk(ws->rescored.As<uint64_t>(), kSel, dCnt + cLp, 1, Q, dK, 0, ws->out.As<uint64_t>(), kmax,
                                 dCnt + cOut, nullptr, s) != 0)
                 return "select after rescore";
             rec(5);
@@ -567,15 +570,16 @@ bool FullRecallSearcher::RunDevice(const VectorFieldData& vf, std::vector<AggrDe
     std::vector<uint32_t> redo;
     uint32_t needCap = cap;
     for (uint32_t q = 0; q < Q; q++) {
+        const uint32_t nCand = hCnt[cCand + static_cast<size_t>(kCandStride) * q];
         m_passing[q] = hCnt[cFilter + q];
-        m_stats.candidates += hCnt[cCand + q];
+        m_stats.candidates += nCand;
         const uint32_t want = std::min(kPrime(m_topKs[q]), m_passing[q]);
-        const bool overflow = hCnt[cCand + q] > cap;
-        const bool shortFloor = floorChecked && std::min(hCnt[cCand + q], cap) < want;
+        const bool overflow = nCand > cap;
+        const bool shortFloor = floorChecked && std::min(nCand, cap) < want;
         if (overflow) m_stats.overflowed++;
         if (overflow || shortFloor) {
             redo.push_back(q);
-            needCap = std::max(needCap, std::min(hCnt[cCand + q], m_passing[q]));
+            needCap = std::max(needCap, std::min(nCand, m_passing[q]));
         }
     }
     for (uint32_t q = 0; q < Q; q++) {

@@ -30,8 +30,8 @@ int AllPassMask(const uint32_t** out) {
 
 int Prepare(const frk_doc_store* store, const float* d_queries, uint32_t n_queries, const uint32_t* d_bmask,
             uint32_t bmask_stride, const float* d_floor, uint64_t block_begin, uint64_t block_step,
-            uint64_t block_count, uint64_t* d_out, uint32_t cap, uint32_t* d_counts, frk_stream_t stream,
-            const char* who, ScoreLaunch& L) {
+            uint64_t block_count, uint64_t* d_out, uint32_t cap, uint32_t* d_counts, uint32_t counts_stride,
+            frk_stream_t stream, const char* who, ScoreLaunch& L) {
     if (!store || !store->tiles || !d_queries || !d_out || (!d_counts && std::string(who) == "frk_score_select")) {
         SetError("%s: null argument", who);
         return FRK_ERR_INVALID;
@@ -84,6 +84,7 @@ int Prepare(const frk_doc_store* store, const float* d_queries, uint32_t n_queri
     A.cand = d_out;
     A.cap = cap;
     A.counts = d_counts;
+    A.counts_stride = counts_stride;
     L.chunks = ch;
     // at most one workgroup per 16 blocks (4 per wave of a 4-wave workgroup; the launch lowers this
     // to the resident count)
@@ -222,11 +223,17 @@ extern "C" uint32_t frk_store_dim(uint32_t dim, uint32_t dtype) {
 extern "C" int frk_score_select(const frk_doc_store* store, const float* d_queries, uint32_t n_queries,
                                 const uint32_t* d_block_masks, uint32_t mask_q_stride, const float* d_floor,
                                 uint64_t block_begin, uint64_t block_step, uint64_t block_count, uint64_t* d_cand,
-                                uint32_t cand_cap, uint32_t* d_counts, frk_stream_t stream) {
+                                uint32_t cand_cap, uint32_t* d_counts, uint32_t counts_stride,
+                                frk_stream_t stream) {
     if (n_queries == 0 || block_count == 0) return FRK_OK;
+    if (counts_stride == 0) {
+        frk::SetError("frk_score_select: counts_stride must be >= 1");
+        return FRK_ERR_INVALID;
+    }
     frk::ScoreLaunch L;
     int rc = frk::Prepare(store, d_queries, n_queries, d_block_masks, mask_q_stride, d_floor, block_begin,
-                          block_step, block_count, d_cand, cand_cap, d_counts, stream, "frk_score_select", L);
+                          block_step, block_count, d_cand, cand_cap, d_counts, counts_stride, stream,
+                          "frk_score_select", L);
     return rc ? rc : frk::Run(store, L, false);
 }
 
@@ -241,7 +248,7 @@ extern "C" int frk_score_blockmax(const frk_doc_store* store, const float* d_que
     }
     frk::ScoreLaunch L;
     int rc = frk::Prepare(store, d_queries, n_queries, d_block_masks, mask_q_stride, nullptr, block_begin,
-                          block_step, block_count, reinterpret_cast<uint64_t*>(d_blockmax), stride, nullptr, stream,
-                          "frk_score_blockmax", L);
+                          block_step, block_count, reinterpret_cast<uint64_t*>(d_blockmax), stride, nullptr, 1,
+                          stream, "frk_score_blockmax", L);
     return rc ? rc : frk::Run(store, L, true);
 }

@@ -123,6 +123,7 @@ struct ScoreArgs {
     uint64_t* cand;
     uint32_t cap;
     uint32_t* counts;
+    uint32_t counts_stride;  // u32 elements between consecutive queries' counters
     uint32_t n_groups;  // query groups of 32 RB rows (set by the launch)
 };
 
@@ -260,7 +261,8 @@ __device__ __forceinline__ void FlushQueue(const ScoreArgs& A, WaveQueue& Qu) {
         uint32_t base = 0;
         if (i < Qu.nr) {
             run = Qu.runs[i];
-            base = atomicAdd(&A.counts[static_cast<uint32_t>(run)], static_cast<uint32_t>(run >> 48));
+            base = atomicAdd(&A.counts[static_cast<uint64_t>(static_cast<uint32_t>(run)) * A.counts_stride],
+                             static_cast<uint32_t>(run >> 48));
         }
         if (i < Qu.nr) {
             const uint32_t q = static_cast<uint32_t>(run);

@@ -71,7 +71,7 @@ __device__ inline void FindDigit(const uint32_t* hist, uint32_t rem, uint32_t* d
 }
 
 __global__ __launch_bounds__(kThreads) void SelectKernel(const uint64_t* __restrict__ cand, uint32_t cap,
-                                                         const uint32_t* __restrict__ counts,
+                                                         const uint32_t* __restrict__ counts, uint32_t counts_stride,
                                                          const uint32_t* __restrict__ ks, uint32_t k_default,
                                                          uint64_t* __restrict__ out, uint32_t k_stride,
                                                          uint32_t* __restrict__ out_counts, float* __restrict__ kth) {
@@ -83,7 +83,8 @@ __global__ __launch_bounds__(kThreads) void SelectKernel(const uint64_t* __restr
     __shared__ uint64_t keys[kLdsKeys];
 
     const uint32_t q = blockIdx.x;
-    const uint32_t n = counts[q] < cap ? counts[q] : cap;
+    const uint32_t cq = counts[static_cast<uint64_t>(q) * counts_stride];
+    const uint32_t n = cq < cap ? cq : cap;
     const uint32_t kq = ks ? (ks[q] < k_stride ? ks[q] : k_stride) : k_default;
     const uint32_t k = kq < n ? kq : n;
     const uint64_t* c = cand + static_cast<uint64_t>(q) * cap;
@@ -276,11 +277,15 @@ extern "C" int frk_kth_largest(const float* d_vals, uint32_t stride, uint32_t n,
 }
 
 extern "C" int frk_select_topk(const uint64_t* d_cand, uint32_t cand_cap, const uint32_t* d_counts,
-                               uint32_t n_queries, const uint32_t* d_k, uint32_t k_default, uint64_t* d_out,
+                               uint32_t counts_stride, uint32_t n_queries, const uint32_t* d_k, uint32_t k_default, uint64_t* d_out,
                                uint32_t k_stride, uint32_t* d_out_counts, float* d_kth_score, frk_stream_t stream) {
     if (n_queries == 0) return FRK_OK;
     if (!d_cand || !d_counts || !d_out || !d_out_counts) {
         frk::SetError("frk_select_topk: null device pointer");
+        return FRK_ERR_INVALID;
+    }
+    if (counts_stride == 0) {
+        frk::SetError("frk_select_topk: counts_stride must be >= 1");
         return FRK_ERR_INVALID;
     }
     if (k_stride > FRK_SELECT_MAX_K || (d_k == nullptr && k_default > k_stride)) {
@@ -289,7 +294,8 @@ extern "C" int frk_select_topk(const uint64_t* d_cand, uint32_t cand_cap, const 
         return FRK_ERR_UNSUPPORTED;
     }
     hipLaunchKernelGGL(frk::SelectKernel, dim3(n_queries), dim3(frk::kThreads), 0, static_cast<hipStream_t>(stream),
-                       d_cand, cand_cap, d_counts, d_k, k_default, d_out, k_stride, d_out_counts, d_kth_score);
+                       d_cand, cand_cap, d_counts, counts_stride, d_k, k_default, d_out, k_stride, d_out_counts,
+                       d_kth_score);
     FRK_HIP_CHECK(hipGetLastError());
     return FRK_OK;
 }

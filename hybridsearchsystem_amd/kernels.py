@@ -75,16 +75,17 @@ def block_masks_host(bits):
 
 
 def score_select(store, d_queries, n_queries, d_block_masks, mask_q_stride, d_floor, block_begin, block_step,
-                 block_count, d_cand, cand_cap, d_counts, stream=None):
+                 block_count, d_cand, cand_cap, d_counts, stream=None, counts_stride=1):
     check(lib().frk_score_select(ctypes.byref(store), ptr(d_queries), n_queries, ptr(d_block_masks), mask_q_stride,
                                  ptr(d_floor), block_begin, block_step, block_count, ptr(d_cand), cand_cap,
-                                 ptr(d_counts), stream_ptr(stream)), "frk_score_select")
+                                 ptr(d_counts), counts_stride, stream_ptr(stream)), "frk_score_select")
 
 
 def select_topk(d_cand, cand_cap, d_counts, n_queries, d_k, k_default, d_out, k_stride, d_out_counts, d_kth=None,
-                stream=None):
-    check(lib().frk_select_topk(ptr(d_cand), cand_cap, ptr(d_counts), n_queries, ptr(d_k), k_default, ptr(d_out),
-                                k_stride, ptr(d_out_counts), ptr(d_kth), stream_ptr(stream)), "frk_select_topk")
+                stream=None, counts_stride=1):
+    check(lib().frk_select_topk(ptr(d_cand), cand_cap, ptr(d_counts), counts_stride, n_queries, ptr(d_k), k_default,
+                                ptr(d_out), k_stride, ptr(d_out_counts), ptr(d_kth), stream_ptr(stream)),
+          "frk_select_topk")
 
 
 def decode_keys(keys):

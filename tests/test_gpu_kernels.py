@@ -155,7 +155,7 @@ def test_filter_rejects_malformed_program(frk):
 
 
 # ------------------------------------------------------------------------------------- scorer + select
-def _run_topk(frk, docs, qs, masks_words, ks, dtype=0, floor=None, block_step=1, cap=None):
+def _run_topk(frk, docs, qs, masks_words, ks, dtype=0, floor=None, block_step=1, cap=None, counts_stride=1):
     n, dim = docs.shape
     tiles, scales, dim_pad = frk.tile_docs(docs, dtype)
     store_t = dev(tiles)
@@ -178,19 +178,25 @@ def _run_topk(frk, docs, qs, masks_words, ks, dtype=0, floor=None, block_step=1,
         assert np.array_equal(d_m.cpu().numpy().view(np.uint32), want)
     cap = cap or n
     d_cand = torch.zeros((nq, cap), dtype=torch.int64, device="cuda")
-    d_cnt = torch.zeros(nq, dtype=torch.int32, device="cuda")
+    # counters counts_stride apart; the words between them must stay untouched
+    d_cnt_all = torch.full((nq * counts_stride,), -7, dtype=torch.int32, device="cuda")
+    d_cnt_all[::counts_stride] = 0
+    d_cnt = d_cnt_all[::counts_stride] if counts_stride > 1 else d_cnt_all
     n_blocks = (n + 31) // 32
     count = (n_blocks + block_step - 1) // block_step
     d_floor = dev(floor.astype(np.float32)) if floor is not None else None
     frk.score_select(store, d_q, nq, d_m, d_m.shape[1] if d_m is not None else 0, d_floor, 0, block_step, count,
-                     d_cand, cap, d_cnt)
+                     d_cand, cap, d_cnt_all, counts_stride=counts_stride)
     kmax = int(max(ks))
     d_k = dev(np.asarray(ks, np.int32))
     d_out = torch.zeros((nq, kmax), dtype=torch.int64, device="cuda")
     d_oc = torch.zeros(nq, dtype=torch.int32, device="cuda")
     d_kth = torch.zeros(nq, dtype=torch.float32, device="cuda")
-    frk.select_topk(d_cand, cap, d_cnt, nq, d_k, 0, d_out, kmax, d_oc, d_kth)
+    frk.select_topk(d_cand, cap, d_cnt_all, nq, d_k, 0, d_out, kmax, d_oc, d_kth, counts_stride=counts_stride)
     torch.cuda.synchronize()
+    if counts_stride > 1:
+        gaps = d_cnt_all.view(nq, counts_stride)[:, 1:]
+        assert bool((gaps == -7).all())
     return (d_out.cpu().numpy().view(np.uint64), d_oc.cpu().numpy(), d_cnt.cpu().numpy(), d_kth.cpu().numpy())
 
 
@@ -220,17 +226,18 @@ def _check_against_truth(frk, docs, qs, bits, ks, keys, oc, rel_tol=1e-5):
                 assert abs(float(d64[d] @ qs[q].astype(np.float64)) - kth) <= 2 * allowed, (q, d)
 
 
-@pytest.mark.parametrize("n,dim,nq", [(5000, 128, 64), (4096 + 17, 256, 100), (1000, 64, 7), (20000, 256, 128),
-                                     (30000, 64, 300)])  # 300 queries: three query groups per doc range
-def test_fp32_topk_matches_cpu_truth(frk, n, dim, nq):
-    print("fp32 topk n=%d dim=%d nq=%d" % (n, dim, nq), flush=True)
+@pytest.mark.parametrize("n,dim,nq,counts_stride", [(5000, 128, 64, 1), (4096 + 17, 256, 100, 1), (1000, 64, 7, 1),
+                                                   (20000, 256, 128, 1), (20000, 256, 128, 32), (1000, 64, 7, 3),
+                                                   (30000, 64, 300, 1)])  # 300 queries: three query groups
+def test_fp32_topk_matches_cpu_truth(frk, n, dim, nq, counts_stride):
+    print("fp32 topk n=%d dim=%d nq=%d counts_stride=%d" % (n, dim, nq, counts_stride), flush=True)
     rng = np.random.default_rng(n + dim + nq)
     docs, cent = _gen.mixture_corpus(rng, n, dim, 16)
     qs = _gen.mixture_queries(rng, cent, nq)
     bits = rng.random((nq, n)) < rng.uniform(0.001, 0.7, (nq, 1))
     ks = rng.choice([1, 10, 100, 1000], size=nq).astype(np.int32)
     masks = np.stack([oracle.words_from_bits(b) for b in bits])
-    keys, oc, cnt, kth = _run_topk(frk, docs, qs, masks, ks)
+    keys, oc, cnt, kth = _run_topk(frk, docs, qs, masks, ks, counts_stride=counts_stride)
     assert np.array_equal(cnt, bits.sum(axis=1))  # unfloored: every passing doc is a candidate
     _check_against_truth(frk, docs, qs, bits, ks, keys, oc)
 
