@@ -19,6 +19,8 @@ typedef float floatx16_t __attribute__((ext_vector_type(16)));
 
 constexpr int kTile = 64;   // C tile per workgroup (4 waves of 32 x 32)
 constexpr int kKStep = 16;  // K per MFMA
+constexpr int kKChunk = 128;  // K staged in LDS per round (a multiple of kKStep)
+static_assert(kKChunk % kKStep == 0 && kTile * kKChunk % 256 == 0, "GEMM staging shape");
 
 __device__ __forceinline__ float Bf16Round(float f) {
     return __builtin_bit_cast(float, static_cast<uint32_t>(F32ToBf16(f)) << 16);
@@ -34,40 +36,53 @@ __global__ __launch_bounds__(256) void GemmKernel(int M, int N, int K, const flo
                                                   const float* __restrict__ B, int ldb, float* __restrict__ C, int ldc,
                                                   int epi, const float* __restrict__ bias, const float* __restrict__ Z,
                                                   int ldz, float* __restrict__ H, int ldh) {
-    __shared__ __attribute__((aligned(16))) uint16_t As[kTile][kKStep];  // [row][k]
-    __shared__ __attribute__((aligned(16))) uint16_t Bs[kTile][kKStep];  // [col][k]
+    // K is staged kKChunk at a time: every load of a round is issued before the first is awaited, so
+    // a layer costs ceil(K / kKChunk) global round trips instead of one per MFMA step (the inference
+    // GEMMs have few workgroups, and their time is those round trips). Rows are padded by 16 bytes so
+    // the 32 rows one MFMA operand read touches start in different LDS banks.
+    __shared__ __attribute__((aligned(16))) uint16_t As[kTile][kKChunk + 8];  // [row][k]
+    __shared__ __attribute__((aligned(16))) uint16_t Bs[kTile][kKChunk + 8];  // [col][k]
     const int t = threadIdx.x;
     const int wave = t >> 6, lane = t & 63;
     const int m0 = blockIdx.y * kTile, n0 = blockIdx.x * kTile;
     const int wm = (wave >> 1) * 32, wn = (wave & 1) * 32;
+    constexpr int kPerThread = kTile * kKChunk / 256;
     floatx16_t acc;
 #pragma unroll
     for (int r = 0; r < 16; r++) acc[r] = 0.0f;
-    for (int k0 = 0; k0 < K; k0 += kKStep) {
+    for (int k0 = 0; k0 < K; k0 += kKChunk) {
+        float va[kPerThread], vb[kPerThread];
 #pragma unroll
-        for (int i = 0; i < 4; i++) {
-            const int idx = t * 4 + i;
+        for (int i = 0; i < kPerThread; i++) {
+            const int idx = t + 256 * i;
             int r, kk;
-            if (TA) { kk = idx / kTile; r = idx % kTile; } else { r = idx / kKStep; kk = idx % kKStep; }
+            if (TA) { kk = idx / kTile; r = idx % kTile; } else { r = idx / kKChunk; kk = idx % kKChunk; }
             const int m = m0 + r, k = k0 + kk;
-            float v = 0.0f;
-            if (m < M && k < K) v = TA ? A[static_cast<int64_t>(k) * lda + m] : A[static_cast<int64_t>(m) * lda + k];
-            As[r][kk] = F32ToBf16(v);
+            va[i] = (m < M && k < K) ? (TA ? A[static_cast<int64_t>(k) * lda + m] : A[static_cast<int64_t>(m) * lda + k])
+                                     : 0.0f;
+            int c;
+            if (TB) { c = idx / kKChunk; kk = idx % kKChunk; } else { kk = idx / kTile; c = idx % kTile; }
+            const int n = n0 + c, kb = k0 + kk;
+            vb[i] = (n < N && kb < K) ? (TB ? B[static_cast<int64_t>(n) * ldb + kb] : B[static_cast<int64_t>(kb) * ldb + n])
+                                      : 0.0f;
         }
 #pragma unroll
-        for (int i = 0; i < 4; i++) {
-            const int idx = t * 4 + i;
-            int c, kk;
-            if (TB) { c = idx / kKStep; kk = idx % kKStep; } else { kk = idx / kTile; c = idx % kTile; }
-            const int n = n0 + c, k = k0 + kk;
-            float v = 0.0f;
-            if (n < N && k < K) v = TB ? B[static_cast<int64_t>(n) * ldb + k] : B[static_cast<int64_t>(k) * ldb + n];
-            Bs[c][kk] = F32ToBf16(v);
+        for (int i = 0; i < kPerThread; i++) {
+            const int idx = t + 256 * i;
+            int r, kk;
+            if (TA) { kk = idx / kTile; r = idx % kTile; } else { r = idx / kKChunk; kk = idx % kKChunk; }
+            As[r][kk] = F32ToBf16(va[i]);
+            int c;
+            if (TB) { c = idx / kKChunk; kk = idx % kKChunk; } else { kk = idx / kTile; c = idx % kTile; }
+            Bs[c][kk] = F32ToBf16(vb[i]);
         }
         __syncthreads();
-        const bf16x8_t a = *reinterpret_cast<const bf16x8_t*>(&As[wm + (lane & 31)][8 * (lane >> 5)]);
-        const bf16x8_t b = *reinterpret_cast<const bf16x8_t*>(&Bs[wn + (lane & 31)][8 * (lane >> 5)]);
-        acc = __builtin_amdgcn_mfma_f32_32x32x16_bf16(a, b, acc, 0, 0, 0);
+        // the same MFMA steps, in the same order, as one step of kKStep per round: K rounded up to kKStep
+        for (int kk = 0; kk < kKChunk && k0 + kk < K; kk += kKStep) {
+            const bf16x8_t a = *reinterpret_cast<const bf16x8_t*>(&As[wm + (lane & 31)][kk + 8 * (lane >> 5)]);
+            const bf16x8_t b = *reinterpret_cast<const bf16x8_t*>(&Bs[wn + (lane & 31)][kk + 8 * (lane >> 5)]);
+            acc = __builtin_amdgcn_mfma_f32_32x32x16_bf16(a, b, acc, 0, 0, 0);
+        }
         __syncthreads();
     }
     // accumulator element r of lane l: row 8 (r / 4) + 4 (l / 32) + r % 4, column l % 32
