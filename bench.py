@@ -341,8 +341,8 @@ def main():
     precision = precision_report(table, runs[:6], head, queries, filters, ks, args) if head.shape[0] == args.docs else None
     learned_out = None
     if not args.no_learned:
-        learned_legs = [("bf16", FRK_DTYPE_BF16, 4, 1), ("fp8", FRK_DTYPE_FP8E4M3, 4, 1), ("fp8", FRK_DTYPE_FP8E4M3, 8, 1),
-                        ("fp8", FRK_DTYPE_FP8E4M3, 4, 2), ("fp8", FRK_DTYPE_FP8E4M3, 8, 2)]
+        learned_legs = [("bf16", FRK_DTYPE_BF16, 4, 1), ("bf16", FRK_DTYPE_BF16, 4, 2), ("fp8", FRK_DTYPE_FP8E4M3, 4, 1),
+                        ("fp8", FRK_DTYPE_FP8E4M3, 8, 1), ("fp8", FRK_DTYPE_FP8E4M3, 4, 2), ("fp8", FRK_DTYPE_FP8E4M3, 8, 2)]
         learned_out, lruns = learned_leg(table, dens, args, queries, filters, ks, res_p, centroids, seed, learned_legs)
         runs += lruns
     # the fastest low-precision pipeline whose recall@k against plain FP32 is >= 0.99

@@ -261,3 +261,40 @@ def test_c2_plain_fp32_5M_docs(search_mod):
             bad += 1
     print("c2: %d of %d queries differ only by near-ties; stats %s" % (bad, nq, st), flush=True)
     assert bad <= 3
+
+
+def test_edge_cases_every_store(search_mod):
+    """Filters that pass nothing (a term absent from the index, a contradiction), fewer passing docs
+    than k (k = 4096 over a rare tag), k = 1, match-all, and a batch of one query, on the FP32, BF16 and
+    FP8 paths: counts equal min(k, passing), an empty result is an empty row, and where every passing
+    doc fits in k each store returns exactly the passing set."""
+    n, dim, n_tags = 20000 + 13, 64, 12
+    docs, dens, off, ids, words = _corpus(31, n, dim, n_tags)
+    qs = synth.mixture_queries(32, 8, dim, 32, corpus_seed=31)
+    rare = int(np.argmin(np.diff(off)))
+    filters = ["424242", "(1 AND NOT 1)", str(rare), None, "(0 OR 1)", str(rare), "2", "(3 AND 4)"]
+    ks = np.array([10, 5, 4096, 1, 100, 1, 1000, 50], np.uint32)
+    valid = [f if f != "424242" else "(1 AND NOT 1)" for f in filters]  # both pass no doc
+    masks, *truth = _truth(docs, words, qs, valid, ks)
+    passing = np.array([int(oracle.bits_from_words(m, n).sum()) for m in masks])
+    assert passing[0] == 0 and passing[1] == 0 and 0 < passing[2] < 4096
+    t = search_mod.Table.build(docs, off, ids, docs_per_segment=8192,
+                               dtypes=(search_mod.FRK_DTYPE_BF16, search_mod.FRK_DTYPE_FP8E4M3))
+    plain = t.search(qs, filters, ks)
+    assert np.array_equal(plain[3], passing)
+    assert _check(docs, qs, masks, ks, plain[0], plain[1], plain[2], truth) <= 1
+    for q in (0, 1):
+        assert plain[2][q] == 0 and np.all(plain[0][q] == -1)
+    for dtype in (search_mod.FRK_DTYPE_BF16, search_mod.FRK_DTYPE_FP8E4M3):
+        print("edge cases dtype", dtype, flush=True)
+        r = t.search(qs, filters, ks, dtype=dtype, rescore_factor=4)
+        assert np.array_equal(r[2], plain[2]) and np.array_equal(r[3], passing)
+        for q in range(len(qs)):
+            got = set(r[0][q, :r[2][q]].tolist())
+            want = set(plain[0][q, :plain[2][q]].tolist())
+            if passing[q] <= ks[q]:
+                assert got == want, q
+    # a batch of one: the same row as in the batch of eight
+    one = t.search(qs[2:3], filters[2:3], ks[2:3])
+    assert one[2][0] == plain[2][2] and np.array_equal(one[0][0, :one[2][0]], plain[0][2, :plain[2][2]])
+    t.free()
