@@ -24,6 +24,8 @@
 #include <stdarg.h>
 #include <stdio.h>
 
+#include <utility>
+
 #include "frk_common.h"
 
 namespace frk {
@@ -34,6 +36,7 @@ constexpr int kMaxSlots = 16;  // S * 512 W <= 8192 words (64 KiB)
 constexpr uint32_t kLayoutEmpty = 15;  // a staged operand with no hits (absent posting or hitCount 0)
 constexpr uint32_t kStatusBadProgram = 1u;
 constexpr uint32_t kStatusTooManyPostings = 2u;
+
 
 struct PostingHead {
     const uint8_t* payload;
@@ -302,51 +305,36 @@ __global__ __launch_bounds__(kThreads) void FilterKernel(const frk_filter_query*
         }
         __syncthreads();
 
-        // ---- phase 3: the program, per word (the program is uniform over the workgroup, so every
-        // thread reaches the same verdict on a malformed one)
+        // ---- phase 3: the program. It is uniform over the workgroup, so its decoding runs on the scalar
+        // unit, and that decoding (repeated per word) bounds this phase. One scalar pass first
+        // validates the program and finds its deepest stack (a malformed one yields all-zero words
+        // and the status flag, the same verdict in every thread); the evaluation then needs no
+        // checks, and a program whose stack stays within kPairDepth entries evaluates two words per
+        // pass, with one 2-word vector per stack entry (a 1-D array indexed by the uniform stack
+        // pointer, so it stays in registers).
         const uint32_t* prog = expr + Q.expr_off;
         bool bad = false;
-#pragma unroll
-        for (int j = 0; j < W; j++) {
-            const uint32_t wl = t + kThreads * j;
-            const uint32_t w = w0 + wl;
-            auto operand = [&](uint32_t p) -> uint64_t {
-                if (p < S) return slot_words[static_cast<size_t>(p) * C + wl];
-                return w < seg_words ? OperandWordDirect(ReadHead(tab[p]), w) : 0;
-            };
-            uint64_t stack[FRK_FILTER_MAX_STACK];
+        int depth = 0;
+        {
             int sp = 0;
             uint32_t pc = 0;
             for (uint32_t i = 0; i < Q.expr_len && !bad; i++) {
                 const uint32_t op = prog[i];
-                uint64_t v;
                 if (op == 0) {  // CONJ
                     if (i + 1 >= Q.expr_len) { bad = true; break; }
                     const uint32_t parts = prog[i + 1];
                     if (i + 1 + parts >= Q.expr_len) { bad = true; break; }
-                    v = ~0ull;
                     for (uint32_t x = 0; x < parts; x++) {
                         const uint32_t n = prog[i + 2 + x];
                         if (pc + n > Q.n_postings) { bad = true; break; }
-                        uint64_t acc = 0;
-                        for (uint32_t y = 0; y < n; y++) acc |= operand(pc + y);
                         pc += n;
-                        v &= acc;
                     }
                     i += parts + 1;
                 } else if (op == 1 || op == 2 || op == 3) {
                     if (i + 2 >= Q.expr_len) { bad = true; break; }
                     const uint32_t pn = prog[i + 1], sn = prog[i + 2];
                     if (pc + pn > Q.n_postings || sn > static_cast<uint32_t>(sp)) { bad = true; break; }
-                    if (op == 3) {
-                        if (pn + sn != 1) { bad = true; break; }
-                        v = ~(pn ? operand(pc) : stack[sp - 1]);
-                    } else {
-                        const bool is_and = op == 1;
-                        v = is_and ? ~0ull : 0ull;
-                        for (uint32_t x = 0; x < pn; x++) v = is_and ? (v & operand(pc + x)) : (v | operand(pc + x));
-                        for (uint32_t x = 0; x < sn; x++) v = is_and ? (v & stack[sp - 1 - x]) : (v | stack[sp - 1 - x]);
-                    }
+                    if (op == 3 && pn + sn != 1) { bad = true; break; }
                     pc += pn;
                     sp -= static_cast<int>(sn);
                     i += 2;
@@ -355,10 +343,82 @@ __global__ __launch_bounds__(kThreads) void FilterKernel(const frk_filter_query*
                     break;
                 }
                 if (sp >= FRK_FILTER_MAX_STACK) { bad = true; break; }
-                stack[sp++] = v;
+                sp++;
+                depth = sp > depth ? sp : depth;
             }
             if (sp != 1 || pc != Q.n_postings) bad = true;
-            result[j] = bad ? 0 : stack[0];
+        }
+        // the evaluation of a validated program: v(e) = op over operands and popped entries, kE words
+        auto evaluate = [&](auto ew_c, auto depth_c, int j0) __attribute__((always_inline)) {
+            constexpr int kE = decltype(ew_c)::value;
+            constexpr int kDepth = decltype(depth_c)::value;
+            typedef uint64_t vec __attribute__((ext_vector_type(kE)));
+            uint32_t wl[kE];
+#pragma unroll
+            for (int e = 0; e < kE; e++) wl[e] = t + kThreads * (j0 + e);
+            auto operand = [&](uint32_t p) -> vec {
+                vec o;
+#pragma unroll
+                for (int e = 0; e < kE; e++) {
+                    if (p < S) {
+                        o[e] = slot_words[static_cast<size_t>(p) * C + wl[e]];
+                    } else {
+                        const uint32_t w = w0 + wl[e];
+                        o[e] = w < seg_words ? OperandWordDirect(ReadHead(tab[p]), w) : 0;
+                    }
+                }
+                return o;
+            };
+            vec stack[kDepth];
+            int sp = 0;
+            uint32_t pc = 0;
+            for (uint32_t i = 0; i < Q.expr_len; i++) {
+                const uint32_t op = prog[i];
+                vec v;
+                if (op == 0) {  // CONJ: AND over parts of the OR of each part's postings
+                    const uint32_t parts = prog[i + 1];
+                    v = ~vec(0);
+                    for (uint32_t x = 0; x < parts; x++) {
+                        const uint32_t n = prog[i + 2 + x];
+                        vec acc = vec(0);
+                        for (uint32_t y = 0; y < n; y++) acc |= operand(pc + y);
+                        pc += n;
+                        v &= acc;
+                    }
+                    i += parts + 1;
+                } else {
+                    const uint32_t pn = prog[i + 1], sn = prog[i + 2];
+                    if (op == 3) {
+                        v = ~(pn ? operand(pc) : stack[sp - 1]);
+                    } else if (op == 1) {
+                        v = ~vec(0);
+                        for (uint32_t x = 0; x < pn; x++) v &= operand(pc + x);
+                        for (uint32_t x = 0; x < sn; x++) v &= stack[sp - 1 - x];
+                    } else {
+                        v = vec(0);
+                        for (uint32_t x = 0; x < pn; x++) v |= operand(pc + x);
+                        for (uint32_t x = 0; x < sn; x++) v |= stack[sp - 1 - x];
+                    }
+                    pc += pn;
+                    sp -= static_cast<int>(sn);
+                    i += 2;
+                }
+                stack[sp++] = v;
+            }
+#pragma unroll
+            for (int e = 0; e < kE; e++) result[j0 + e] = stack[0][e];
+        };
+        constexpr int kPairDepth = FRK_FILTER_MAX_STACK / 2;
+        if (!bad) {
+            if (W >= 2 && depth <= kPairDepth) {
+#pragma unroll
+                for (int j0 = 0; j0 < W; j0 += 2)
+                    evaluate(std::integral_constant<int, W >= 2 ? 2 : 1>{}, std::integral_constant<int, kPairDepth>{}, j0);
+            } else {
+#pragma unroll
+                for (int j0 = 0; j0 < W; j0++)
+                    evaluate(std::integral_constant<int, 1>{}, std::integral_constant<int, FRK_FILTER_MAX_STACK>{}, j0);
+            }
         }
         if (bad) {
             if (t == 0) atomicOr(status, kStatusBadProgram);
