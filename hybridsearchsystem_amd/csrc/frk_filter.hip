@@ -179,7 +179,7 @@ __device__ uint64_t OperandWordDirect(const PostingHead& h, uint32_t w) {
 }
 
 template <int W>
-__global__ __launch_bounds__(kThreads) void FilterKernel(const frk_filter_query* __restrict__ queries,
+__global__ __launch_bounds__(kThreads, 4) void FilterKernel(const frk_filter_query* __restrict__ queries,
                                                          const uint32_t* __restrict__ expr,
                                                          const uint64_t* __restrict__ table, uint32_t seg_docs,
                                                          uint64_t n_docs, uint64_t* __restrict__ out,
@@ -409,8 +409,11 @@ __global__ __launch_bounds__(kThreads) void FilterKernel(const frk_filter_query*
             for (int e = 0; e < kE; e++) result[j0 + e] = stack[0][e];
         };
         constexpr int kPairDepth = FRK_FILTER_MAX_STACK / 2;
+        constexpr int kQuadDepth = FRK_FILTER_MAX_STACK / 4;
         if (!bad) {
-            if (W >= 2 && depth <= kPairDepth) {
+            if (W == 4 && depth <= kQuadDepth) {
+                evaluate(std::integral_constant<int, W == 4 ? 4 : 1>{}, std::integral_constant<int, kQuadDepth>{}, 0);
+            } else if (W >= 2 && depth <= kPairDepth) {
 #pragma unroll
                 for (int j0 = 0; j0 < W; j0 += 2)
                     evaluate(std::integral_constant<int, W >= 2 ? 2 : 1>{}, std::integral_constant<int, kPairDepth>{}, j0);
