@@ -34,7 +34,8 @@ def dev(a, dtype=None):
     return t.cuda()
 
 
-def _filter_case(frk, seed, n_docs, seg_docs, n_tags, n_queries, depth, all_layouts=True, max_postings=0):
+def _filter_case(frk, seed, n_docs, seg_docs, n_tags, n_queries, depth, all_layouts=True, max_postings=0,
+                 filters=None):
     print("filter case seed=%d n_docs=%d seg=%d tags=%d q=%d" % (seed, n_docs, seg_docs, n_tags, n_queries),
           flush=True)
     rng = np.random.default_rng(seed)
@@ -50,7 +51,9 @@ def _filter_case(frk, seed, n_docs, seg_docs, n_tags, n_queries, depth, all_layo
     exprs, tables, descs, asts = [], [], [], []
     expr_off = table_off = 0
     for qi in range(n_queries):
-        if qi == 0:
+        if filters is not None:
+            ast = oracle.parse_filter(filters[qi])
+        elif qi == 0:
             ast = np.zeros((0, 2), np.int64)  # match all
         elif qi == 1:
             ast = oracle.parse_filter(str(int(rng.integers(0, n_tags))))  # a bare term
@@ -105,6 +108,27 @@ def test_filter_bit_exact_full_segments(frk, max_postings):
     # programs' posting counts sends the extra operands down the direct path
     _filter_case(frk, seed=2, n_docs=131072 + 19264, seg_docs=131072, n_tags=24, n_queries=20, depth=3,
                  max_postings=max_postings)
+
+
+def _nested_filter(levels, first_tag=0):
+    """((a AND b) OR ((c AND d) OR ... (y AND z))): a postfix program whose stack grows one entry per
+    level (levels + 1 deep)."""
+    t = first_tag
+    e = "(%d AND %d)" % (t, t + 1)
+    for i in range(levels):
+        e = "((%d AND %d) OR %s)" % (t + 2 * i + 2, t + 2 * i + 3, e)
+    return e
+
+
+@pytest.mark.parametrize("max_postings", [0, 4])
+def test_filter_deep_programs(frk, max_postings):
+    # stacks 2 to 12 deep: the four-word (<= 4), two-word (<= 8) and one-word evaluation passes of a
+    # whole-segment chunk (max_postings 4: 4 slots, later operands read directly), and the one-word
+    # pass of the 16-slot launch (max_postings 0)
+    filters = [_nested_filter(k, first_tag=k % 3) for k in (1, 3, 4, 5, 7, 8, 9, 11)]
+    filters += ["NOT " + _nested_filter(6), "(" + _nested_filter(3) + " AND NOT " + _nested_filter(9) + ")"]
+    _filter_case(frk, seed=7, n_docs=131072 + 4096 + 17, seg_docs=131072, n_tags=26, n_queries=len(filters),
+                 depth=0, all_layouts=False, max_postings=max_postings, filters=filters)
 
 
 def test_filter_single_segment_tiny(frk):
