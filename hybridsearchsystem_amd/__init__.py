@@ -52,6 +52,8 @@ def lib():
     L.frk_score_select.argtypes = [ctypes.POINTER(DocStore), vp, u32, vp, u32, vp, u64, u64, u64, vp, u32, vp, u32, vp]
     L.frk_select_topk.restype = i32
     L.frk_select_topk.argtypes = [vp, u32, vp, u32, u32, vp, u32, vp, u32, vp, vp, vp]
+    L.frk_select_top_set.restype = i32
+    L.frk_select_top_set.argtypes = [vp, u32, vp, u32, u32, vp, u32, vp, u32, vp, vp, vp]
     _bind_optional(L)
     _LIB = L
     return L

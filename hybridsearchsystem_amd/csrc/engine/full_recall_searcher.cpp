@@ -469,7 +469,8 @@ bool FullRecallSearcher::RunDevice(const VectorFieldData& vf, std::vector<AggrDe
             rec(4);
             rec(5);
         } else {
-            if (frk_select_topk(ws->cand.As<uint64_t>(), cap, dCnt + cCand, kCandStride, Q, dKm, 0, ws->lp.As<uint64_t>(), kSel,
+            // the k*m best by the low-precision score, unsorted: they are rescored and sorted after
+            if (frk_select_top_set(ws->cand.As<uint64_t>(), cap, dCnt + cCand, kCandStride, Q, dKm, 0, ws->lp.As<uint64_t>(), kSel,
                                 dCnt + cLp, nullptr, s) != 0)
                 return "select k*m";
             rec(4);

@@ -70,6 +70,9 @@ __device__ inline void FindDigit(const uint32_t* hist, uint32_t rem, uint32_t* d
     }
 }
 
+// kSorted: the k keys sorted descending; otherwise the same k keys in no particular order (what a
+// caller that rescores them needs), which skips the final sort.
+template <bool kSorted>
 __global__ __launch_bounds__(kThreads) void SelectKernel(const uint64_t* __restrict__ cand, uint32_t cap,
                                                          const uint32_t* __restrict__ counts, uint32_t counts_stride,
                                                          const uint32_t* __restrict__ ks, uint32_t k_default,
@@ -96,7 +99,12 @@ __global__ __launch_bounds__(kThreads) void SelectKernel(const uint64_t* __restr
         }
         return;
     }
-    if (n <= FRK_SELECT_MAX_K) {
+    if (!kSorted && k == n && !kth) {
+        for (uint32_t i = threadIdx.x; i < n; i += kThreads) o[i] = c[i];
+        if (threadIdx.x == 0) out_counts[q] = k;
+        return;
+    }
+    if (kSorted && n <= FRK_SELECT_MAX_K) {
         // few candidates: sort them all (padding keys 0 sort last)
         uint32_t p2 = 1;
         while (p2 < n) p2 <<= 1;
@@ -193,21 +201,28 @@ __global__ __launch_bounds__(kThreads) void SelectKernel(const uint64_t* __restr
     if (threadIdx.x == 0) s_fill = 0;
     __syncthreads();
     const uint64_t T = prefix;  // the k-th largest key
-    // the k keys >= T: those of bins above the chosen digits were never compacted, so read them all
+    // the k keys >= T (keys are unique, so exactly k): those of bins above the chosen digits were
+    // never compacted, so read them all
     for (uint32_t i = threadIdx.x; i < n; i += kThreads) {
         const uint64_t key = c[i];
         if (key >= T) {
             const uint32_t at = atomicAdd(&s_fill, 1u);
-            if (at < FRK_SELECT_MAX_K) sorted[at] = key;
+            if (kSorted) {
+                if (at < FRK_SELECT_MAX_K) sorted[at] = key;
+            } else if (at < k) {
+                o[at] = key;
+            }
         }
     }
-    __syncthreads();
-    uint32_t p2 = 1;
-    while (p2 < k) p2 <<= 1;
-    for (uint32_t i = k + threadIdx.x; i < p2; i += kThreads) sorted[i] = 0;
-    __syncthreads();
-    BitonicSortDesc(sorted, p2);
-    for (uint32_t i = threadIdx.x; i < k; i += kThreads) o[i] = sorted[i];
+    if constexpr (kSorted) {
+        __syncthreads();
+        uint32_t p2 = 1;
+        while (p2 < k) p2 <<= 1;
+        for (uint32_t i = k + threadIdx.x; i < p2; i += kThreads) sorted[i] = 0;
+        __syncthreads();
+        BitonicSortDesc(sorted, p2);
+        for (uint32_t i = threadIdx.x; i < k; i += kThreads) o[i] = sorted[i];
+    }
     if (threadIdx.x == 0) {
         out_counts[q] = k;
         if (kth) kth[q] = (kq <= n) ? KeyScore(T) : -INFINITY;
@@ -276,26 +291,47 @@ extern "C" int frk_kth_largest(const float* d_vals, uint32_t stride, uint32_t n,
     return FRK_OK;
 }
 
-extern "C" int frk_select_topk(const uint64_t* d_cand, uint32_t cand_cap, const uint32_t* d_counts,
-                               uint32_t counts_stride, uint32_t n_queries, const uint32_t* d_k, uint32_t k_default, uint64_t* d_out,
-                               uint32_t k_stride, uint32_t* d_out_counts, float* d_kth_score, frk_stream_t stream) {
+static int SelectLaunch(const char* who, bool sorted_out, const uint64_t* d_cand, uint32_t cand_cap,
+                        const uint32_t* d_counts, uint32_t counts_stride, uint32_t n_queries, const uint32_t* d_k,
+                        uint32_t k_default, uint64_t* d_out, uint32_t k_stride, uint32_t* d_out_counts,
+                        float* d_kth_score, frk_stream_t stream) {
     if (n_queries == 0) return FRK_OK;
     if (!d_cand || !d_counts || !d_out || !d_out_counts) {
-        frk::SetError("frk_select_topk: null device pointer");
+        frk::SetError("%s: null device pointer", who);
         return FRK_ERR_INVALID;
     }
     if (counts_stride == 0) {
-        frk::SetError("frk_select_topk: counts_stride must be >= 1");
+        frk::SetError("%s: counts_stride must be >= 1", who);
         return FRK_ERR_INVALID;
     }
     if (k_stride > FRK_SELECT_MAX_K || (d_k == nullptr && k_default > k_stride)) {
-        frk::SetError("frk_select_topk: k_stride %u / k %u beyond the supported %d", k_stride, k_default,
-                      FRK_SELECT_MAX_K);
+        frk::SetError("%s: k_stride %u / k %u beyond the supported %d", who, k_stride, k_default, FRK_SELECT_MAX_K);
         return FRK_ERR_UNSUPPORTED;
     }
-    hipLaunchKernelGGL(frk::SelectKernel, dim3(n_queries), dim3(frk::kThreads), 0, static_cast<hipStream_t>(stream),
-                       d_cand, cand_cap, d_counts, counts_stride, d_k, k_default, d_out, k_stride, d_out_counts,
-                       d_kth_score);
+    if (sorted_out)
+        hipLaunchKernelGGL(frk::SelectKernel<true>, dim3(n_queries), dim3(frk::kThreads), 0,
+                           static_cast<hipStream_t>(stream), d_cand, cand_cap, d_counts, counts_stride, d_k, k_default,
+                           d_out, k_stride, d_out_counts, d_kth_score);
+    else
+        hipLaunchKernelGGL(frk::SelectKernel<false>, dim3(n_queries), dim3(frk::kThreads), 0,
+                           static_cast<hipStream_t>(stream), d_cand, cand_cap, d_counts, counts_stride, d_k, k_default,
+                           d_out, k_stride, d_out_counts, d_kth_score);
     FRK_HIP_CHECK(hipGetLastError());
     return FRK_OK;
+}
+
+extern "C" int frk_select_topk(const uint64_t* d_cand, uint32_t cand_cap, const uint32_t* d_counts,
+                               uint32_t counts_stride, uint32_t n_queries, const uint32_t* d_k, uint32_t k_default,
+                               uint64_t* d_out, uint32_t k_stride, uint32_t* d_out_counts, float* d_kth_score,
+                               frk_stream_t stream) {
+    return SelectLaunch("frk_select_topk", true, d_cand, cand_cap, d_counts, counts_stride, n_queries, d_k, k_default,
+                        d_out, k_stride, d_out_counts, d_kth_score, stream);
+}
+
+extern "C" int frk_select_top_set(const uint64_t* d_cand, uint32_t cand_cap, const uint32_t* d_counts,
+                                  uint32_t counts_stride, uint32_t n_queries, const uint32_t* d_k, uint32_t k_default,
+                                  uint64_t* d_out, uint32_t k_stride, uint32_t* d_out_counts, float* d_kth_score,
+                                  frk_stream_t stream) {
+    return SelectLaunch("frk_select_top_set", false, d_cand, cand_cap, d_counts, counts_stride, n_queries, d_k,
+                        k_default, d_out, k_stride, d_out_counts, d_kth_score, stream);
 }

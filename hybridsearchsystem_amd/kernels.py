@@ -88,6 +88,13 @@ def select_topk(d_cand, cand_cap, d_counts, n_queries, d_k, k_default, d_out, k_
           "frk_select_topk")
 
 
+def select_top_set(d_cand, cand_cap, d_counts, n_queries, d_k, k_default, d_out, k_stride, d_out_counts, d_kth=None,
+                   stream=None, counts_stride=1):
+    check(lib().frk_select_top_set(ptr(d_cand), cand_cap, ptr(d_counts), counts_stride, n_queries, ptr(d_k), k_default,
+                                   ptr(d_out), k_stride, ptr(d_out_counts), ptr(d_kth), stream_ptr(stream)),
+          "frk_select_top_set")
+
+
 def decode_keys(keys):
     """uint64 candidate keys -> (scores float32, doc ids int64)."""
     keys = np.asarray(keys, dtype=np.uint64)

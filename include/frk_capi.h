@@ -175,6 +175,13 @@ int frk_select_topk(const uint64_t* d_cand, uint32_t cand_cap, const uint32_t* d
                     uint32_t n_queries,
                     const uint32_t* d_k, uint32_t k_default, uint64_t* d_out, uint32_t k_stride,
                     uint32_t* d_out_counts, float* d_kth_score, frk_stream_t stream);
+/* The same selection without the final sort: out[q * k_stride ..] holds the best min(k_q, n) keys in no
+ * particular order (out_counts and kth_score as above). For callers that process the set further, such
+ * as FP32 rescoring of the top k * m before the final sorted select; no reference counterpart (the
+ * reference sorts on the host, result_aggregator.cpp:993-1000). */
+int frk_select_top_set(const uint64_t* d_cand, uint32_t cand_cap, const uint32_t* d_counts, uint32_t counts_stride,
+                       uint32_t n_queries, const uint32_t* d_k, uint32_t k_default, uint64_t* d_out, uint32_t k_stride,
+                       uint32_t* d_out_counts, float* d_kth_score, frk_stream_t stream);
 
 /* ------------------------------------------------------------------------------------------------
  * FP32 rescoring of candidates after a low-precision pass: keys_out[q][i] = MakeKey(<q, doc>, doc)

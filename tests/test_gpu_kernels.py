@@ -403,6 +403,18 @@ def test_select_and_kth_largest_at_every_path_boundary(frk):
         frk.select_topk(d_c, cap, d_n, nq, d_k, 0, d_out, kv, d_oc, d_kth)
         torch.cuda.synchronize()
         out, oc, kth = d_out.cpu().numpy().view(np.uint64), d_oc.cpu().numpy(), d_kth.cpu().numpy()
+        # the unsorted set selection: the same keys in any order, the same counts and k-th scores
+        # (without a k-th output where k >= n, the copy-through path)
+        d_out2 = torch.zeros((nq, kv), dtype=torch.int64, device="cuda")
+        d_oc2 = torch.zeros(nq, dtype=torch.int32, device="cuda")
+        d_kth2 = torch.zeros(nq, dtype=torch.float32, device="cuda")
+        frk.select_top_set(d_c, cap, d_n, nq, d_k, 0, d_out2, kv, d_oc2, d_kth2)
+        d_out3 = torch.zeros((nq, kv), dtype=torch.int64, device="cuda")
+        d_oc3 = torch.zeros(nq, dtype=torch.int32, device="cuda")
+        frk.select_top_set(d_c, cap, d_n, nq, d_k, 0, d_out3, kv, d_oc3, None)
+        torch.cuda.synchronize()
+        out2, oc2, kth2 = d_out2.cpu().numpy().view(np.uint64), d_oc2.cpu().numpy(), d_kth2.cpu().numpy()
+        out3, oc3 = d_out3.cpu().numpy().view(np.uint64), d_oc3.cpu().numpy()
         for q, n in enumerate(sizes):
             want = np.sort(keys[q, :n])[::-1][:kv]
             assert oc[q] == len(want), (kv, n)
@@ -411,6 +423,10 @@ def test_select_and_kth_largest_at_every_path_boundary(frk):
                 assert kth[q] == frk.decode_keys(want[-1:])[0][0]
             else:
                 assert kth[q] == -np.inf
+            assert oc2[q] == oc3[q] == len(want), (kv, n)
+            assert np.array_equal(np.sort(out2[q, :oc2[q]]), np.sort(want)), (kv, n)
+            assert np.array_equal(np.sort(out3[q, :oc3[q]]), np.sort(want)), (kv, n)
+            assert kth2[q] == kth[q], (kv, n)
     for n in (100, 16384, 16385, 40000):
         vals = rng.standard_normal((3, n)).astype(np.float32)
         vals[1, : n // 3] = 0.25
