@@ -3,11 +3,12 @@
 //
 // One batch is one stream-ordered device pipeline with a single synchronisation at the end:
 //   H2D of the packed batch (programs, operand tables, queries, k) -> filter (all queries) ->
-//   [probe: score a strided sample of doc blocks, k-th best per query = the score floor] ->
-//   fused score + mask + floor + compaction over all docs -> exact top-k select ->
-//   [low precision: FP32 rescore of the top k*m, select again] -> D2H of the top-k keys.
+//   [probe: score a strided sample of doc blocks, k-th best per query = the score floor, or the
+//   learned floor model] -> fused score + mask + floor + compaction over all docs -> exact top-k
+//   select -> [low precision: the top k*m as a set, FP32 rescore, sorted select] -> D2H of the
+//   top-k keys.
 // The reference instead syncs after the scorer, per query after the filter and per query after
-// each aggregation step (SURVEY call stack S2).
+// each aggregation step (full_recall_searcher.cpp:107-231, result_aggregator.cpp:363-777).
 #pragma once
 
 #include <hip/hip_runtime.h>
