@@ -367,7 +367,8 @@ def main():
     learned_out = None
     if not args.no_learned:
         learned_legs = [("bf16", FRK_DTYPE_BF16, 4, 1), ("bf16", FRK_DTYPE_BF16, 4, 2), ("fp8", FRK_DTYPE_FP8E4M3, 4, 1),
-                        ("fp8", FRK_DTYPE_FP8E4M3, 8, 1), ("fp8", FRK_DTYPE_FP8E4M3, 4, 2), ("fp8", FRK_DTYPE_FP8E4M3, 8, 2)]
+                        ("fp8", FRK_DTYPE_FP8E4M3, 8, 1), ("fp8", FRK_DTYPE_FP8E4M3, 4, 2), ("fp8", FRK_DTYPE_FP8E4M3, 8, 2),
+                        ("fp8", FRK_DTYPE_FP8E4M3, 4, 3), ("fp8", FRK_DTYPE_FP8E4M3, 8, 3)]
         learned_out, lruns = learned_leg(table, dens, args, queries, filters, ks, res_p, centroids, seed, learned_legs)
         runs += lruns
     # the fastest low-precision pipeline whose recall@k against plain FP32 is >= 0.99
