@@ -269,6 +269,14 @@ def test_learned_floor_is_exact_and_prunes():
         for i, what in ((0, "ids"), (1, "scores"), (2, "counts")):
             bad = np.nonzero(np.any(np.atleast_2d(base[i] != got[i]).reshape(len(base[2]), -1), axis=1))[0]
             assert len(bad) == 0, "%s: %s differ for queries %s" % (name, what, bad[:10])
+    # the learned floor inside a captured graph: replays equal the unpruned results too
+    replayed = 0
+    for rep in range(3):
+        g = t.search(qs[te], ft, ks[te], probe=False, floor_model=model, graph=True)
+        for i, what in ((0, "ids"), (1, "scores"), (2, "counts")):
+            assert np.array_equal(base[i], g[i]), "graph run %d: %s differ" % (rep, what)
+        replayed += g[4]["graph_replayed"]
+    assert replayed >= 1
     gain = base[4]["candidates"] / max(dev[4]["candidates"], 1)
     print("candidates without floor %d, learned floor %d on the host / %d on the device (x%.1f), reruns %d / %d "
           "of %d, margin %.4f" % (base[4]["candidates"], host[4]["candidates"], dev[4]["candidates"], gain,

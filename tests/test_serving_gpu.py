@@ -23,7 +23,8 @@ def corpus():
     rng = np.random.default_rng(62)
     dens = synth.loguniform_densities(rng, n_tags, lo=0.001, hi=0.6)
     off, ids = synth.gen_postings(rng, n, dens)
-    t = search.Table.build(docs, off, ids, docs_per_segment=65536, dtypes=(search.FRK_DTYPE_BF16,))
+    t = search.Table.build(docs, off, ids, docs_per_segment=65536,
+                           dtypes=(search.FRK_DTYPE_BF16, search.FRK_DTYPE_FP8E4M3))
     qs = synth.mixture_queries(63, 2048, dim, 64, corpus_seed=61)
     fs, _ = synth.random_filters(64, dens, 2048, depth=3, sel=(0.0005, 0.6))
     yield t, qs, fs
@@ -34,7 +35,7 @@ def _same(a, b):
     return np.array_equal(a[0], b[0]) and np.array_equal(a[1], b[1]) and np.array_equal(a[2], b[2])
 
 
-@pytest.mark.parametrize("dtype", [0, 1])
+@pytest.mark.parametrize("dtype", [0, 1, 2])
 def test_graph_replay_matches_eager(corpus, dtype):
     t, qs, fs = corpus
     ks = np.random.default_rng(65).choice([10, 100], size=96).astype(np.uint32)
