@@ -40,6 +40,9 @@ constexpr uint32_t kXcds = 8;  // accelerator complex dies of an MI355X, each wi
 #ifndef FRK_SCORE_WIDE_TILE
 #define FRK_SCORE_WIDE_TILE 64  // query tiles above this many KiB get 8-wave workgroups
 #endif
+#ifndef FRK_SCORE_WIDE_TILE_FP8
+#define FRK_SCORE_WIDE_TILE_FP8 16  // the same for FP8 stores
+#endif
 
 // Workgroup size of a shape: 4 waves, or 8 where the query tile (RB x CH KiB of LDS) is so large that
 // a CU holds only one workgroup; 8 waves then still give each SIMD two waves, one streaming its doc
@@ -49,11 +52,15 @@ constexpr uint32_t kXcds = 8;  // accelerator complex dies of an MI355X, each wi
 // 2.83 -> 3.41 ms; 1024 dims: 1.82 -> 2.31 ms on 5M docs).
 template <int DT, int CH, int RB>
 struct WgShape {
-    // (FP8 at a 128 KiB tile would spill under the 256-register cap of two waves per SIMD)
-    static constexpr int kThreads = DT != FRK_DTYPE_BF16 && RB * CH > FRK_SCORE_WIDE_TILE &&
-                                            RB * CH <= (DT == FRK_DTYPE_FP8E4M3 ? 96 : 128)
-                                        ? 512
-                                        : 256;
+    // (FP8 at a 128 KiB tile would spill under the 256-register cap of two waves per SIMD). FP8 shapes
+    // also go wide from a 16 KiB tile on: their register count caps them at two waves per SIMD
+    // anyway, and one 8-wave workgroup per CU with one tile measured faster than two 4-wave ones with a
+    // tile each (10M x 256 FP8, 128 queries, in the pipeline: 0.626 -> 0.606 ms).
+    static constexpr int kThreads =
+        DT != FRK_DTYPE_BF16 && RB * CH > (DT == FRK_DTYPE_FP8E4M3 ? FRK_SCORE_WIDE_TILE_FP8 : FRK_SCORE_WIDE_TILE) &&
+                RB * CH <= (DT == FRK_DTYPE_FP8E4M3 ? 96 : 128)
+            ? 512
+            : 256;
     static constexpr int kWaves = kThreads / 64;
     // Query fragments staged in registers one group ahead: a 4-wave workgroup whose query tile leaves
     // room for only one workgroup per CU runs a single wave per SIMD, which has no other wave to cover

@@ -285,15 +285,15 @@ def _dequantized(x, dtype):
 
 
 @pytest.mark.parametrize("dtype,dim,nq", [(1, 128, 70), (2, 128, 70), (1, 768, 150), (2, 768, 150), (0, 384, 150),
-                                          (0, 384, 90), (2, 1536, 150)])
+                                          (0, 384, 90), (2, 1536, 150), (2, 256, 128), (2, 512, 64)])
 def test_low_precision_scores_match_dequantized_values(frk, dtype, dim, nq):
     """Every filter-passing doc comes out once with the dot product of the stored (dequantized) query and
     doc values. Doc and query magnitudes span six decades, so a per-lane scale applied to the wrong row
     or column of the scaled FP8 MFMA shows; 3,000 docs leave a ragged last block, 70 queries three row
     blocks of one query group. The wider cases run the other workgroup shapes: 48 chunks per doc (BF16
     768, FP32 384, FP8 1536 dims) with three row blocks per group and query fragments staged in
-    registers (150 queries: two groups, the second partly empty; 90: one), FP8 768 dims with 8-wave
-    workgroups."""
+    registers (150 queries: two groups, the second partly empty; 90: one), FP8 with 8-wave workgroups
+    (768 dims; 256 dims at 128 queries, the benchmark's shape; 512 dims at 64)."""
     print("low precision scores dtype %d dim %d nq %d" % (dtype, dim, nq), flush=True)
     rng = np.random.default_rng(60 + dtype if (dim, nq) == (128, 70) else 60 + dtype + dim + nq)
     n = 3000
