@@ -41,23 +41,25 @@ constexpr uint32_t kXcds = 8;  // accelerator complex dies of an MI355X, each wi
 #define FRK_SCORE_WIDE_TILE 64  // query tiles above this many KiB get 8-wave workgroups
 #endif
 #ifndef FRK_SCORE_WIDE_TILE_FP8
-#define FRK_SCORE_WIDE_TILE_FP8 16  // the same for FP8 stores
+#define FRK_SCORE_WIDE_TILE_FP8 16  // the same for FP8 and BF16 stores
 #endif
 
 // Workgroup size of a shape: 4 waves, or 8 where the query tile (RB x CH KiB of LDS) is so large that
 // a CU holds only one workgroup; 8 waves then still give each SIMD two waves, one streaming its doc
 // tiles while the other runs its MFMAs. All waves of a workgroup share its one query tile. Measured on
 // 10M docs: FP8 768 dims 1.56 -> 1.51 ms at 128 queries and 2.30 -> 2.05 ms at 256, FP32 256 dims
-// 5.12 -> 4.99 ms at 128; BF16 is kept at 4 waves, where 8 measured slower (768 dims, 128 queries:
-// 2.83 -> 3.41 ms; 1024 dims: 1.82 -> 2.31 ms on 5M docs).
+// 5.12 -> 4.99 ms at 128; BF16 tiles above 64 KiB are kept at 4 waves, where 8 measured slower (768
+// dims, 128 queries: 2.83 -> 3.41 ms; 1024 dims: 1.82 -> 2.31 ms on 5M docs).
 template <int DT, int CH, int RB>
 struct WgShape {
-    // (FP8 at a 128 KiB tile would spill under the 256-register cap of two waves per SIMD). FP8 shapes
-    // also go wide from a 16 KiB tile on: their register count caps them at two waves per SIMD
-    // anyway, and one 8-wave workgroup per CU with one tile measured faster than two 4-wave ones with a
-    // tile each (10M x 256 FP8, 128 queries, in the pipeline: 0.626 -> 0.606 ms).
+    // (FP8 at a 128 KiB tile would spill under the 256-register cap of two waves per SIMD). FP8 and
+    // BF16 shapes also go wide from a 16 KiB tile on (BF16 up to 64 KiB): their register counts cap
+    // them at two waves per SIMD anyway, and one 8-wave workgroup per CU with one tile measured faster
+    // than two 4-wave ones with a tile each (10M x 256, 128 queries: FP8 in the pipeline 0.626 -> 0.606
+    // ms, BF16 scorer alone 0.94 -> 0.91 ms).
     static constexpr int kThreads =
-        DT != FRK_DTYPE_BF16 && RB * CH > (DT == FRK_DTYPE_FP8E4M3 ? FRK_SCORE_WIDE_TILE_FP8 : FRK_SCORE_WIDE_TILE) &&
+        (DT != FRK_DTYPE_BF16 || RB * CH <= 64) &&
+                RB * CH > (DT == FRK_DTYPE_F32 ? FRK_SCORE_WIDE_TILE : FRK_SCORE_WIDE_TILE_FP8) &&
                 RB * CH <= (DT == FRK_DTYPE_FP8E4M3 ? 96 : 128)
             ? 512
             : 256;
