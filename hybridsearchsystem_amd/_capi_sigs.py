@@ -27,6 +27,8 @@ class IndexPostingField(ctypes.Structure):
 
 
 def bind(L):
+    L.frk_check_program.restype = i32
+    L.frk_check_program.argtypes = [vp, u32, u32, vp]
     L.frk_tiles_bytes.restype = u64
     L.frk_tiles_bytes.argtypes = [u64, u32, u32]
     L.frk_tile_docs.restype = i32

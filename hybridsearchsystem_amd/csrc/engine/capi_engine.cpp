@@ -238,8 +238,8 @@ int frk_floor_features(const frk_table* t, const char* posting_field, uint32_t n
 }
 
 namespace {
-// AddQuery x n + BatchSearch over prebuilt trees; results are decoded straight from the sorted keys
-// (score from the ordered high word, global id through the id mapping), as BatchSearchExtract does.
+// AddQuery x n + BatchSearch over prebuilt trees; the results are FullRecallSearcher's own
+// (BatchSearchExtract: scores from the keys, global ids through the id mapping).
 int SearchTrees(const frk_table* t, const char* vector_field, uint32_t n_queries, const float* queries, uint32_t dim,
                 QueryNode* const* trees, const uint32_t* ks, const frk_search_options* opts, uint32_t k_stride,
                 int64_t* out_ids, float* out_scores, uint32_t* out_counts, uint32_t* out_passing,
@@ -261,21 +261,19 @@ int SearchTrees(const frk_table* t, const char* vector_field, uint32_t n_queries
         o.graph = opts->graph != 0;
         searcher.SetOptions(o);
     }
+    // BatchSearchDevice + BatchSearchExtract, the two halves of BatchSearch, so the passing counts the
+    // device half reports are at hand for the caller as well
     std::vector<AggrDeviceResult> dev;
     if (!searcher.BatchSearchDevice(vector_field, false, dev)) return FRK_ERR_INVALID;
-    const DocIdMapping* map = t->table->GetDocIdMapping();
+    std::vector<std::shared_ptr<FullRecallResult>> results;
+    if (!searcher.BatchSearchExtract(dev, results, false) || results.size() != n_queries)
+        return Fail(FRK_ERR_INVALID, "frk_search: result extraction failed");
     for (uint32_t q = 0; q < n_queries; q++) {
-        const std::vector<uint64_t>& keys = dev[q].keys;
-        int64_t* ids = out_ids + static_cast<size_t>(q) * k_stride;
-        float* sc = out_scores ? out_scores + static_cast<size_t>(q) * k_stride : nullptr;
-        for (size_t i = 0; i < keys.size(); i++) {
-            const uint32_t ldoc = 0xFFFFFFFFu - static_cast<uint32_t>(keys[i]);
-            uint64_t gdoc = ldoc;
-            if (map && !map->GetGDocId(ldoc, gdoc)) return Fail(FRK_ERR_INVALID, "frk_search: doc without a global id");
-            ids[i] = static_cast<int64_t>(gdoc);
-            if (sc) sc[i] = frk::KeyScore(keys[i]);
-        }
-        out_counts[q] = static_cast<uint32_t>(keys.size());
+        const FullRecallResult& r = *results[q];
+        const size_t n = r.docIds.size();
+        std::memcpy(out_ids + static_cast<size_t>(q) * k_stride, r.docIds.data(), n * sizeof(int64_t));
+        if (out_scores) std::memcpy(out_scores + static_cast<size_t>(q) * k_stride, r.scores.data(), n * sizeof(float));
+        out_counts[q] = static_cast<uint32_t>(n);
         if (out_passing) out_passing[q] = dev[q].passing;
     }
     if (stats) {

@@ -1,6 +1,8 @@
 // query_node.cpp -- query tree, postfix compilation and the filter grammar (see query_node.h).
 #include "query_node.h"
 
+#include "../frk_filter_program.h"
+
 #include <algorithm>
 #include <cmath>
 #include <mutex>
@@ -82,7 +84,7 @@ bool NotNode::GetPostOrderExpression(const DataTable& t, std::vector<uint32_t>& 
 
 bool TermNode::IsNodeValid(const std::unordered_set<std::string>& postingFields) const {
     if (!postingFields.count(m_fieldName)) {
-        FRK_LOG_WARN("field[" << m_fieldName << "] not exist in index, will be ignored.");
+        FRK_LOG_WARN("filter term dropped: posting field '" << m_fieldName << "' is not in this index");
         return false;
     }
     return true;
@@ -104,7 +106,7 @@ bool TermNode::AppendPostings(const DataTable& t, PostingOperands& a) const {
 
 bool TermsNode::IsNodeValid(const std::unordered_set<std::string>& postingFields) const {
     if (!postingFields.count(m_fieldName)) {
-        FRK_LOG_WARN("field[" << m_fieldName << "] not exist in index, will be ignored.");
+        FRK_LOG_WARN("filter term dropped: posting field '" << m_fieldName << "' is not in this index");
         return false;
     }
     return true;
@@ -118,7 +120,7 @@ bool TermsNode::GetPostOrderExpression(const DataTable& t, std::vector<uint32_t>
     if (!AppendPostings(t, a)) return false;
     const uint32_t postingNum = static_cast<uint32_t>(a.size() - base);
     if (postingNum == 0) {
-        FRK_LOG_ERROR("field:" << m_fieldName << " has no valid token.");
+        FRK_LOG_ERROR("terms node over posting field '" << m_fieldName << "' resolved no token");
         return false;
     }
     e.push_back(OR);
@@ -148,7 +150,7 @@ bool OrTermsNode::GetPostOrderExpression(const DataTable& t, std::vector<uint32_
     if (!AppendPostings(t, a)) return false;
     const uint32_t postingNum = static_cast<uint32_t>(a.size() - base);
     if (postingNum == 0) {
-        FRK_LOG_ERROR("or_terms has no valid token.");
+        FRK_LOG_ERROR("or-terms node resolved no token");
         return false;
     }
     e.push_back(OR);
@@ -185,7 +187,7 @@ bool ConjunctionNode::GetPostOrderExpression(const DataTable& t, std::vector<uin
         if (num > 0) groups.push_back(num);
     }
     if (groups.empty()) {
-        FRK_LOG_ERROR("Conjunction has no valid token.");
+        FRK_LOG_ERROR("conjunction node resolved no token");
         return false;
     }
     e.push_back(CONJUNCTION);
@@ -196,36 +198,12 @@ bool ConjunctionNode::GetPostOrderExpression(const DataTable& t, std::vector<uin
 }
 
 // ------------------------------------------------------------------------------------------ prepare
+// The structural rule of the reference's check (text_filter.cpp:1220-1352), in the wrap-free form the
+// filter kernel applies too (frk_filter_program.h); any stack depth is allowed here, the kernel's
+// limit is checked separately with its own message.
 bool CheckPostExpr(const std::vector<uint32_t>& e, uint32_t postingNums) {
-    uint32_t inStack = 0, remained = postingNums;
-    for (size_t i = 0; i < e.size(); i++) {
-        const uint32_t op = e[i];
-        if (op == CONJUNCTION) {
-            if (i + 1 >= e.size()) return false;
-            const uint32_t parts = e[i + 1];
-            if (parts == 0 || i + 1 + parts >= e.size()) return false;
-            uint32_t sum = 0;
-            for (uint32_t j = 0; j < parts; j++) {
-                if (e[i + 2 + j] == 0) return false;
-                sum += e[i + 2 + j];
-            }
-            if (remained < sum) return false;
-            remained -= sum;
-            inStack += 1;
-            i += parts + 1;
-        } else if (op == AND || op == OR || op == NOT) {
-            if (i + 2 >= e.size()) return false;
-            const uint32_t p = e[i + 1], s = e[i + 2];
-            if (op == NOT ? p + s != 1 : p + s == 0) return false;
-            if (remained < p || inStack < s) return false;
-            inStack = inStack - s + 1;
-            remained -= p;
-            i += 2;
-        } else {
-            return false;
-        }
-    }
-    return remained == 0 && inStack == 1;
+    if (e.size() > 0xFFFFFFFFull) return false;
+    return frk::ValidateProgram(e.data(), static_cast<uint32_t>(e.size()), postingNums, nullptr, 1 << 30);
 }
 
 uint32_t PostExprMaxStack(const std::vector<uint32_t>& e) {
@@ -369,11 +347,11 @@ bool PrepareFilterExpr(const DataTable& table, QueryNode* tree, FilterExpr& out)
         out.emptyTree = out.valid = true;  // text_filter.cpp:443-448
         return true;
     } else if (!tree->GetPostOrderExpression(table, out.postExpr, out.postingDeviceAddrs, out.opNum)) {
-        FRK_LOG_ERROR("get post order expression failed");
+        FRK_LOG_ERROR("postfix emission of the filter tree failed");
         return false;
     }
     if (!CheckPostExpr(out.postExpr, static_cast<uint32_t>(out.postingDeviceAddrs.size()))) {
-        FRK_LOG_ERROR("CheckPostExpr failed");
+        FRK_LOG_ERROR("compiled filter program failed the structural check");
         return false;
     }
     if (PostExprMaxStack(out.postExpr) > FRK_FILTER_MAX_STACK ||

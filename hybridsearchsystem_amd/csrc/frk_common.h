@@ -12,6 +12,9 @@ namespace frk {
 // Thread-local last-error text, returned by frk_last_error().
 void SetError(const char* fmt, ...);
 
+// Compute units of the calling thread's current device, queried once per device and cached.
+int CurrentDeviceCUs(int* cus);
+
 // Maps a failed HIP call to FRK_ERR_HIP with the call site in the error text.
 #define FRK_HIP_CHECK(expr)                                                              \
     do {                                                                                 \

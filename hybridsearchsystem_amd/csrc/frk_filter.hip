@@ -27,6 +27,7 @@
 #include <utility>
 
 #include "frk_common.h"
+#include "frk_filter_program.h"
 
 namespace frk {
 namespace {
@@ -313,41 +314,8 @@ __global__ __launch_bounds__(kThreads, 4) void FilterKernel(const frk_filter_que
         // pass, with one 2-word vector per stack entry (a 1-D array indexed by the uniform stack
         // pointer, so it stays in registers).
         const uint32_t* prog = expr + Q.expr_off;
-        bool bad = false;
         int depth = 0;
-        {
-            int sp = 0;
-            uint32_t pc = 0;
-            for (uint32_t i = 0; i < Q.expr_len && !bad; i++) {
-                const uint32_t op = prog[i];
-                if (op == 0) {  // CONJ
-                    if (i + 1 >= Q.expr_len) { bad = true; break; }
-                    const uint32_t parts = prog[i + 1];
-                    if (i + 1 + parts >= Q.expr_len) { bad = true; break; }
-                    for (uint32_t x = 0; x < parts; x++) {
-                        const uint32_t n = prog[i + 2 + x];
-                        if (pc + n > Q.n_postings) { bad = true; break; }
-                        pc += n;
-                    }
-                    i += parts + 1;
-                } else if (op == 1 || op == 2 || op == 3) {
-                    if (i + 2 >= Q.expr_len) { bad = true; break; }
-                    const uint32_t pn = prog[i + 1], sn = prog[i + 2];
-                    if (pc + pn > Q.n_postings || sn > static_cast<uint32_t>(sp)) { bad = true; break; }
-                    if (op == 3 && pn + sn != 1) { bad = true; break; }
-                    pc += pn;
-                    sp -= static_cast<int>(sn);
-                    i += 2;
-                } else {
-                    bad = true;
-                    break;
-                }
-                if (sp >= FRK_FILTER_MAX_STACK) { bad = true; break; }
-                sp++;
-                depth = sp > depth ? sp : depth;
-            }
-            if (sp != 1 || pc != Q.n_postings) bad = true;
-        }
+        bool bad = !ValidateProgram(prog, Q.expr_len, Q.n_postings, &depth);
         // the evaluation of a validated program: v(e) = op over operands and popped entries, kE words
         auto evaluate = [&](auto ew_c, auto depth_c, int j0) __attribute__((always_inline)) {
             constexpr int kE = decltype(ew_c)::value;
@@ -495,17 +463,21 @@ extern "C" int frk_filter_eval(const frk_filter_query* d_queries, uint32_t n_que
     const size_t lds = static_cast<size_t>(slots) * chunk * sizeof(uint64_t);
     dim3 grid((seg_words + chunk - 1) / chunk, n_segments, n_queries);
     hipStream_t st = static_cast<hipStream_t>(stream);
-    static bool lds_set = false;  // allow 64 KiB of dynamic LDS for every instantiation, once
-    if (!lds_set) {
+    // allow 64 KiB of dynamic LDS for every instantiation, once per process (a function-local static
+    // is initialised exactly once even when several host threads launch at the same time)
+    static const hipError_t lds_set = [] {
         const int max_lds = frk::kMaxSlots * frk::kThreads * static_cast<int>(sizeof(uint64_t));
-        FRK_HIP_CHECK(hipFuncSetAttribute(reinterpret_cast<const void*>(&frk::FilterKernel<4>),
-                                          hipFuncAttributeMaxDynamicSharedMemorySize, max_lds));
-        FRK_HIP_CHECK(hipFuncSetAttribute(reinterpret_cast<const void*>(&frk::FilterKernel<2>),
-                                          hipFuncAttributeMaxDynamicSharedMemorySize, max_lds));
-        FRK_HIP_CHECK(hipFuncSetAttribute(reinterpret_cast<const void*>(&frk::FilterKernel<1>),
-                                          hipFuncAttributeMaxDynamicSharedMemorySize, max_lds));
-        lds_set = true;
-    }
+        hipError_t e = hipFuncSetAttribute(reinterpret_cast<const void*>(&frk::FilterKernel<4>),
+                                           hipFuncAttributeMaxDynamicSharedMemorySize, max_lds);
+        if (e == hipSuccess)
+            e = hipFuncSetAttribute(reinterpret_cast<const void*>(&frk::FilterKernel<2>),
+                                    hipFuncAttributeMaxDynamicSharedMemorySize, max_lds);
+        if (e == hipSuccess)
+            e = hipFuncSetAttribute(reinterpret_cast<const void*>(&frk::FilterKernel<1>),
+                                    hipFuncAttributeMaxDynamicSharedMemorySize, max_lds);
+        return e;
+    }();
+    FRK_HIP_CHECK(lds_set);
     switch (w) {
         case 4:
             hipLaunchKernelGGL(frk::FilterKernel<4>, grid, dim3(frk::kThreads), lds, st, d_queries, d_expr, d_table,
@@ -521,5 +493,19 @@ extern "C" int frk_filter_eval(const frk_filter_query* d_queries, uint32_t n_que
             break;
     }
     FRK_HIP_CHECK(hipGetLastError());
+    return FRK_OK;
+}
+
+extern "C" int frk_check_program(const uint32_t* expr, uint32_t expr_len, uint32_t n_postings, uint32_t* max_depth) {
+    if (!expr && expr_len) {
+        frk::SetError("frk_check_program: null program");
+        return FRK_ERR_INVALID;
+    }
+    int depth = 0;
+    if (n_postings > FRK_FILTER_MAX_POSTINGS || !frk::ValidateProgram(expr, expr_len, n_postings, &depth)) {
+        frk::SetError("frk_check_program: malformed program (%u words, %u postings)", expr_len, n_postings);
+        return FRK_ERR_INVALID;
+    }
+    if (max_depth) *max_depth = static_cast<uint32_t>(depth);
     return FRK_OK;
 }

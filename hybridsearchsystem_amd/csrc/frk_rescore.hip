@@ -11,7 +11,8 @@ typedef float floatx4 __attribute__((ext_vector_type(4)));
 // One wave per candidate. Lane l reads the 16-byte (chunk l / 2, half l % 2) piece of the doc's
 // tile column: 4 dims, so a wave covers 256 dims per step.
 __global__ __launch_bounds__(256) void RescoreKernel(const floatx4* __restrict__ tiles, uint32_t chunks,
-                                                     const float* __restrict__ queries, uint32_t q_dim,
+                                                     uint64_t n_docs, const float* __restrict__ queries,
+                                                     uint32_t q_dim,
                                                      const uint64_t* __restrict__ keys_in, uint32_t stride,
                                                      const uint32_t* __restrict__ counts,
                                                      uint64_t* __restrict__ keys_out) {
@@ -22,6 +23,10 @@ __global__ __launch_bounds__(256) void RescoreKernel(const floatx4* __restrict__
     if (i >= n) return;
     const uint64_t key = keys_in[static_cast<uint64_t>(q) * stride + i];
     const uint32_t doc = KeyDoc(key);
+    if (doc >= n_docs) {  // not a doc of this store: the smallest key, which no selection prefers
+        if (lane == 0) keys_out[static_cast<uint64_t>(q) * stride + i] = 0;
+        return;
+    }
     const uint64_t b = doc / 32, j = doc % 32;
     const float* qv = queries + static_cast<uint64_t>(q) * q_dim;
     float acc = 0.0f;
@@ -73,8 +78,10 @@ __global__ __launch_bounds__(256) void RescoreRowsKernel(const float* __restrict
 #pragma unroll
     for (int c = 0; c < kPer; c++)
         for (int off = 32; off > 0; off >>= 1) acc[c] += __shfl_xor(acc[c], off, 64);
-    if (lane < kPer && ok[lane])
-        keys_out[static_cast<uint64_t>(q) * stride + i0 + lane] = MakeKey(acc[lane], doc[lane]);
+    // a key whose doc lies outside the table becomes 0, the smallest key, which no selection prefers
+    // (so the count stays valid and no stale key of an earlier call is read back)
+    if (lane < kPer && i0 + lane < n)
+        keys_out[static_cast<uint64_t>(q) * stride + i0 + lane] = ok[lane] ? MakeKey(acc[lane], doc[lane]) : 0ull;
 }
 
 }  // namespace
@@ -95,7 +102,8 @@ extern "C" int frk_rescore(const frk_doc_store* f32_store, const float* d_querie
         return FRK_ERR_UNSUPPORTED;
     }
     hipLaunchKernelGGL(frk::RescoreKernel, grid, dim3(256), 0, static_cast<hipStream_t>(stream),
-                       static_cast<const frk::floatx4*>(f32_store->tiles), chunks, d_queries, f32_store->dim,
+                       static_cast<const frk::floatx4*>(f32_store->tiles), chunks, f32_store->n_docs, d_queries,
+                       f32_store->dim,
                        d_keys_in, stride, d_counts, d_keys_out);
     FRK_HIP_CHECK(hipGetLastError());
     return FRK_OK;

@@ -1,5 +1,6 @@
 // frk_score.hip -- C ABI of the fused scorer: argument checks, launch shape, dtype dispatch. The
 // kernel is in frk_score_impl.h, instantiated per dtype in frk_score_{f32,bf16,fp8}.hip.
+#include <atomic>
 #include <string>
 
 #include "frk_score_impl.h"
@@ -12,19 +13,21 @@ __device__ uint32_t kAllPassMask[1] = {0xFFFFFFFFu};
 
 int AllPassMask(const uint32_t** out) {
     constexpr int kMaxDevices = 64;
-    static const uint32_t* addr[kMaxDevices] = {};
+    static std::atomic<const uint32_t*> addr[kMaxDevices] = {};  // racing threads store the same address
     int dev = 0;
     FRK_HIP_CHECK(hipGetDevice(&dev));
     if (dev < 0 || dev >= kMaxDevices) {
         SetError("device %d out of range", dev);
         return FRK_ERR_UNSUPPORTED;
     }
-    if (!addr[dev]) {
+    const uint32_t* a = addr[dev].load(std::memory_order_relaxed);
+    if (!a) {
         void* p = nullptr;
         FRK_HIP_CHECK(hipGetSymbolAddress(&p, HIP_SYMBOL(kAllPassMask)));
-        addr[dev] = static_cast<const uint32_t*>(p);
+        a = static_cast<const uint32_t*>(p);
+        addr[dev].store(a, std::memory_order_relaxed);
     }
-    *out = addr[dev];
+    *out = a;
     return FRK_OK;
 }
 

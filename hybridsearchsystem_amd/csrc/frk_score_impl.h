@@ -630,16 +630,23 @@ int Launch(const ScoreArgs& A, hipStream_t stream, uint32_t grid_x) {
     const size_t lds = static_cast<size_t>(RB) * CH * 64 * sizeof(u32x4) + 2 * 32 * RB * sizeof(float) +
                        kWaves * (kQueueKeys + kQueueRuns) * sizeof(uint64_t);
     const void* fn = reinterpret_cast<const void*>(&ScoreKernel<DT, CH, RB, kBlockMax>);
-    static int resident_per_cu = 0;  // workgroups of this shape one CU holds at once
-    if (resident_per_cu == 0) {
-        FRK_HIP_CHECK(hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize, static_cast<int>(lds)));
+    // workgroups of this shape one CU holds at once, with the LDS attribute set: once per process (a
+    // function-local static is initialised exactly once even when several host threads launch together)
+    struct Residency {
+        hipError_t err;
+        int per_cu;
+    };
+    static const Residency res = [&] {
+        Residency r{hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize, static_cast<int>(lds)), 1};
         int occ = 0;
-        FRK_HIP_CHECK(hipOccupancyMaxActiveBlocksPerMultiprocessor(&occ, fn, kThreads, lds));
-        resident_per_cu = occ > 0 ? occ : 1;
-    }
-    int dev = 0, cus = 0;
-    FRK_HIP_CHECK(hipGetDevice(&dev));
-    FRK_HIP_CHECK(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev));
+        if (r.err == hipSuccess) r.err = hipOccupancyMaxActiveBlocksPerMultiprocessor(&occ, fn, kThreads, lds);
+        r.per_cu = occ > 0 ? occ : 1;
+        return r;
+    }();
+    FRK_HIP_CHECK(res.err);
+    const int resident_per_cu = res.per_cu;
+    int cus = 0;
+    if (int rc = CurrentDeviceCUs(&cus)) return rc;
     const uint32_t groups = (A.n_queries + 32 * RB - 1) / (32 * RB);
     // one wave of resident workgroups over the chip (each walks a contiguous block range, so the
     // query tile is staged once per workgroup), never more than the caller's work-based cap

@@ -77,6 +77,13 @@ int frk_filter_eval(const frk_filter_query* d_queries, uint32_t n_queries, const
                     uint64_t* d_out, uint64_t out_stride, uint32_t* d_counts, uint32_t* d_status,
                     uint32_t max_postings, frk_stream_t stream);
 
+/* Host check of one program (TextFilter::CheckPostExpr, engine/retrieval/searcher/stage/text_filter.cpp:
+ * 1220-1352), the same rule the filter kernel applies before it evaluates a program: every op's words
+ * exist, postings are consumed in order within n_postings, only stack entries are popped, NOT takes one
+ * operand, and the program ends with every posting consumed and one entry on the stack (no sum of
+ * program words may wrap). FRK_OK and *max_depth (optional) = the deepest stack, or FRK_ERR_INVALID. */
+int frk_check_program(const uint32_t* expr, uint32_t expr_len, uint32_t n_postings, uint32_t* max_depth);
+
 /* ------------------------------------------------------------------------------------------------
  * Doc store: vectors in the MFMA-native tiled layout. Docs are grouped in blocks of 32; the dims in
  * chunks of 16 bytes per lane (f32: 4 dims, bf16: 8, fp8: 16) for two lane halves, so a chunk is
@@ -185,7 +192,8 @@ int frk_select_top_set(const uint64_t* d_cand, uint32_t cand_cap, const uint32_t
 
 /* ------------------------------------------------------------------------------------------------
  * FP32 rescoring of candidates after a low-precision pass: keys_out[q][i] = MakeKey(<q, doc>, doc)
- * for doc = the doc of keys_in[q][i], i < min(counts[q], stride), computed from the FP32 store.
+ * for doc = the doc of keys_in[q][i], i < min(counts[q], stride), computed from the FP32 store; a key
+ * whose doc lies outside the store becomes 0 (the smallest key, which no selection prefers).
  * d_queries: [n_queries][f32_store->dim] FP32. No reference counterpart (the reference scores FP16
  * only, vector_scorer_mmad.cpp:75-211).
  * ------------------------------------------------------------------------------------------------ */

@@ -578,15 +578,17 @@ int orc_compile_postfix(const int64_t* ast, int n_pairs, uint32_t* expr, int cap
  * entries, and the program ends with all postings consumed and exactly one stack entry. Returns 0
  * for a valid program. */
 int orc_check_postfix(const uint32_t* expr, int len, uint32_t n_postings) {
-    uint32_t in_stack = 0, remained = n_postings;
+    /* Counts are summed in 64 bits: the reference's 32-bit sums (text_filter.cpp:1230,1260,1291) would
+     * let a count near 2^32 wrap past the checks; the rule itself is unchanged. */
+    uint64_t in_stack = 0, remained = n_postings;
     for (int i = 0; i < len; i++) {
         uint32_t op = expr[i];
         if (op == ORC_OP_CONJ) {
             if (i + 1 >= len) return -1;
-            uint32_t parts = expr[i + 1];
-            if (parts == 0 || i + 1 + (int)parts >= len) return -1;
-            uint32_t sum = 0;
-            for (uint32_t j = 0; j < parts; j++) {
+            uint64_t parts = expr[i + 1];
+            if (parts == 0 || (uint64_t)i + 1 + parts >= (uint64_t)len) return -1;
+            uint64_t sum = 0;
+            for (uint64_t j = 0; j < parts; j++) {
                 if (expr[i + 2 + j] == 0) return -1;
                 sum += expr[i + 2 + j];
             }
@@ -596,7 +598,7 @@ int orc_check_postfix(const uint32_t* expr, int len, uint32_t n_postings) {
             i += (int)parts + 1;
         } else if (op == ORC_OP_AND || op == ORC_OP_OR || op == ORC_OP_NOT) {
             if (i + 2 >= len) return -1;
-            uint32_t pn = expr[i + 1], sn = expr[i + 2];
+            uint64_t pn = expr[i + 1], sn = expr[i + 2];
             if (op == ORC_OP_NOT ? (pn + sn != 1) : (pn + sn == 0)) return -1;
             if (remained < pn || in_stack < sn) return -1;
             in_stack -= sn;

@@ -140,3 +140,56 @@ def test_prepared_filters_parse_once_and_reject_bad_text(L):
     for bad in ("(3 AND", "3 OR OR 4", "AND"):
         with pytest.raises(FrkError, match="does not parse"):
             search.PreparedFilter(bad)
+
+
+def _check_program(L, prog, n_postings):
+    import ctypes
+    p = np.ascontiguousarray(prog, dtype=np.uint32)
+    depth = ctypes.c_uint32(0)
+    rc = L.frk_check_program(p.ctypes.data if len(p) else None, len(p), n_postings, ctypes.byref(depth))
+    return rc == 0, int(depth.value)
+
+
+def test_program_check_follows_check_post_expr_and_never_wraps(L):
+    """frk_check_program (the rule the filter kernel applies before evaluating a program) accepts every
+    compiled program with the stack depth the program reaches, agrees with the oracle's CheckPostExpr on
+    single-word mutations of them, rejects the reference's own reject cases (text_filter.cpp:1220-1352),
+    and rejects programs whose counts would wrap a 32-bit sum: a posting count of 0xFFFFFFFF after a real
+    operand, a CONJ part count that runs past the program, a NOT whose two counts sum to 1 mod 2^32."""
+    from tests import _gen
+    rng = np.random.default_rng(17)
+    agree = 0
+    for _ in range(300):
+        f = _gen.rand_filter(rng, 24, 4)
+        prog, posts, _ = oracle.compile_postfix(oracle.parse_filter(f))
+        ok, depth = _check_program(L, prog, len(posts))
+        assert ok and oracle.check_postfix(prog, len(posts)), f
+        assert depth == oracle.postfix_max_stack(prog), f
+        for _ in range(4):
+            bad = prog.copy()
+            i = int(rng.integers(0, len(bad)))
+            bad[i] = int(rng.choice([0, 1, 2, 3, 4, int(bad[i]) + 1, max(int(bad[i]) - 1, 0), 0xFFFFFFFF]))
+            np_ = len(posts) + int(rng.integers(-1, 2))
+            if np_ < 0:
+                continue
+            assert _check_program(L, bad, np_)[0] == oracle.check_postfix(bad, np_), (f, bad.tolist(), np_)
+            agree += 1
+    assert agree > 1000
+    # the reference's reject cases
+    for prog, n in (([1, 0, 0], 0), ([0, 0], 0), ([0, 1, 0], 0), ([3, 1, 1], 1), ([3, 0, 0], 0), ([5, 1, 0], 1),
+                    ([2, 1], 1), ([2, 2, 0], 1), ([2, 1, 0], 2), ([2, 1, 0, 1, 0, 2], 1), ([2, 1, 0, 2, 1, 0], 2), ([], 0)):
+        assert not _check_program(L, prog, n)[0] and not oracle.check_postfix(prog, n), (prog, n)
+    # counts that wrap a 32-bit sum
+    M = 0xFFFFFFFF
+    for prog, n in (([2, 1, 0, 2, M, 1], 1),          # OR over one posting, then OR with 2^32 - 1 more
+                    ([0, 2, 1, M, 2, 0, 1], 1),        # CONJ parts 1 + (2^32 - 1) postings
+                    ([0, M, 1, 1], 2),                 # CONJ whose part count runs past the program
+                    ([2, 1, 0, 3, M, 2], 1),           # NOT with postingNum + stackNum = 1 mod 2^32
+                    ([2, 2, 0, 1, M, 2], 2)):          # AND popping 2 with 2^32 - 1 postings
+        assert not _check_program(L, prog, n)[0], prog
+        assert not oracle.check_postfix(prog, n), prog
+    # and valid programs at the stack limit are accepted with their depth
+    deep = [2, 1, 0] * 16 + [1, 0, 16]
+    assert _check_program(L, deep, 16) == (True, 16)
+    deep = [2, 1, 0] * 17 + [1, 0, 17]  # a 17th entry exceeds the kernel's 16-entry stack
+    assert not _check_program(L, deep, 17)[0] and oracle.check_postfix(deep, 17)

@@ -178,6 +178,27 @@ def test_filter_rejects_malformed_program(frk):
     assert int(d_out.abs().sum().item()) == 0
 
 
+M32 = 0xFFFFFFFF
+
+
+@pytest.mark.parametrize("prog,n_post", [([2, 1, 0, 2, M32, 1], 1), ([0, 2, 1, M32, 2, 0, 1], 1),
+                                         ([2, 1, 0, 3, M32, 2], 1), ([0, M32, 1, 1], 2)])
+def test_filter_rejects_programs_whose_counts_wrap(frk, prog, n_post):
+    """Programs whose counts would pass a 32-bit sum check by wrapping (the same ones the CPU test of
+    frk_check_program rejects): the kernel's check rejects them too, sets the status flag and writes
+    all-zero words, without evaluating a single op."""
+    print("wrapping program", prog, flush=True)
+    d_q = dev(frk.filter_queries_array([(0, len(prog), 0, n_post, 2, 0)]))
+    d_out = torch.full((1, 2), -1, dtype=torch.int64, device="cuda")
+    d_cnt = torch.zeros(1, dtype=torch.int32, device="cuda")
+    d_status = torch.zeros(1, dtype=torch.int32, device="cuda")
+    frk.filter_eval(d_q, 1, dev(np.array(prog, np.uint32).view(np.int32)), dev(np.zeros(n_post, np.int64)), 1, 128,
+                    100, d_out, 2, d_cnt, d_status)
+    torch.cuda.synchronize()
+    assert int(d_status.item()) != 0
+    assert int(d_out.abs().sum().item()) == 0 and int(d_cnt.item()) == 0
+
+
 # ------------------------------------------------------------------------------------- scorer + select
 def _run_topk(frk, docs, qs, masks_words, ks, dtype=0, floor=None, block_step=1, cap=None, counts_stride=1):
     n, dim = docs.shape

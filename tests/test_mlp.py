@@ -29,6 +29,33 @@ def test_oracle_gradients_match_finite_differences():
             assert abs((lp - lm) / 2e-6 - g[k][i]) < 1e-7, (k, i)
 
 
+def test_oracle_pinball_minimiser_is_the_tau_quantile():
+    """What the mathematics fixes about the pinball loss, independent of how the oracle writes it: over
+    a sample of n labels, the constant prediction that minimises the mean loss at level tau is the
+    tau-quantile of the sample, the order statistic s[ceil(tau n) - 1] (the subgradient
+    (#(l < y) - tau n) / n changes sign there). A tau <-> 1 - tau swap, a sign error in the residual or
+    a dropped (tau - 1) factor each move the minimiser to another order statistic."""
+    rng = np.random.default_rng(3)
+    n = 2001
+    lab = rng.standard_normal(n) * 2.0 + 0.5
+    s = np.sort(lab)
+    for tau in (0.02, 0.3, 0.9):
+        j = int(np.ceil(tau * n)) - 1
+        assert s[j] == np.quantile(lab, tau, method="inverted_cdf")
+        losses = np.array([omlp.pinball(np.full(n, y), lab, tau)[0] for y in s])
+        assert int(np.argmin(losses)) == j, tau
+        # the loss is piecewise linear with these slopes on either side of the minimiser
+        for a, b, slope in ((s[j - 1], s[j], j / n - tau), (s[j], s[j + 1], (j + 1) / n - tau)):
+            y, eps = 0.5 * (a + b), 0.4 * (b - a)  # the loss is linear between neighbouring labels
+            fd = (omlp.pinball(np.full(n, y + eps), lab, tau)[0] - omlp.pinball(np.full(n, y - eps), lab, tau)[0]) / (2 * eps)
+            assert abs(fd - slope) < 1e-5, (tau, fd, slope)
+            # the returned gradient (per row, mean over rows) sums to the same slope
+            assert abs(omlp.pinball(np.full(n, y), lab, tau)[1].sum() - slope) < 1e-12
+    # a single residual: tau * r above the label, (1 - tau) * |r| below
+    assert np.isclose(omlp.pinball([0.0], [2.0], 0.1)[0], 0.2)
+    assert np.isclose(omlp.pinball([2.0], [0.0], 0.1)[0], 1.8)
+
+
 def test_oracle_bf16_rounding_matches_torch():
     x = np.concatenate([np.random.default_rng(1).standard_normal(100000).astype(np.float32) * 10.0 ** np.arange(-5, 5).repeat(10000),
                         np.array([1.00390625, 1.005859375, -1.00390625, 65504.0, np.inf, -np.inf], np.float32)])
@@ -179,6 +206,31 @@ def test_small_kernels_match_reference():
     mh = (0.1 * gr) / 0.1
     vh = (0.001 * gr * gr) / 0.001
     assert np.allclose(p.cpu().numpy(), p0 - 0.01 * mh / (np.sqrt(vh) + 1e-8), rtol=1e-5, atol=1e-6)
+
+
+@pytest.mark.gpu
+def test_pinball_kernel_minimiser_is_the_tau_quantile():
+    """frk_mlp_pinball against what fixes the pinball loss independently of the oracle: for a constant
+    prediction y, the summed gradient is (#(labels < y) - tau n) / n, so it changes sign at the
+    tau-quantile of the labels, and the mean loss there is below its value one order statistic away."""
+    L = _gpu()
+    rng = np.random.default_rng(12)
+    n = 1001
+    lab = rng.standard_normal(n).astype(np.float32)
+    s = np.sort(lab)
+    g = torch.zeros(n, device="cuda")
+    lo = torch.zeros(n, device="cuda")
+    for tau in (0.05, 0.7):
+        j = int(np.ceil(tau * n)) - 1
+        means = []
+        for y in (s[j - 1], s[j], s[j + 1]):
+            mid_lo = float(y) - 1e-4 * float(s[j + 1] - s[j - 1])
+            L.pinball(n, _dev(np.full(n, mid_lo, np.float32)), _dev(lab), tau, g, lo)
+            below = int((lab < mid_lo).sum())
+            assert np.isclose(float(g.sum().item()), (below - tau * n) / n, atol=1e-5), (tau, y)
+            L.pinball(n, _dev(np.full(n, y, np.float32)), _dev(lab), tau, g, lo)
+            means.append(float(lo.mean().item()))
+        assert means[1] < means[0] and means[1] < means[2], (tau, means)
 
 
 @pytest.mark.gpu

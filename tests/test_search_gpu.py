@@ -145,7 +145,7 @@ def test_compiled_program_matches_oracle(search_mod):
     assert t.compile_filter("")[3]
 
 
-@pytest.mark.parametrize("dtype,min_recall", [(1, 0.99), (2, 0.95)])
+@pytest.mark.parametrize("dtype,min_recall", [(1, 0.99), (2, 0.99)])
 def test_low_precision_with_fp32_rescoring(search_mod, dtype, min_recall):
     n, dim, n_tags = 100_000, 256, 40
     docs, dens, off, ids, words = _corpus(31, n, dim, n_tags, centroids=64)
@@ -172,7 +172,7 @@ def test_low_precision_with_fp32_rescoring(search_mod, dtype, min_recall):
 def test_any_dimension(search_mod, dim):
     """Dims that are not a compiled chunk count are zero-padded to the next one (12- and 24-chunk
     shapes cover 96/192/384/768-dim stores without padding): plain FP32 against the CPU truth, BF16 and
-    FP8 with FP32 rescoring against its recall, over 100 queries (4 row blocks per workgroup where the
+    FP8 with FP32 rescoring of the top k*4 (FP8 also k*8) at recall@k >= 0.99 against it, over 100 queries (4 row blocks per workgroup where the
     query tile fits)."""
     from hybridsearchsystem_amd import kernels
     n, n_tags, nq = 8000 + 45, 20, 100
@@ -187,10 +187,11 @@ def test_any_dimension(search_mod, dim):
     r = t.search(qs, filters, ks)
     assert _check(docs, qs, masks, ks, r[0], r[1], r[2], truth) <= 2
     ids32, _, cnt32 = truth
-    for dtype, min_recall in ((1, 0.99), (2, 0.95)):
-        lo = t.search(qs, filters, ks, dtype=dtype, rescore_factor=4)
+    for dtype, m in ((1, 4), (2, 4), (2, 8)):
+        lo = t.search(qs, filters, ks, dtype=dtype, rescore_factor=m)
         hits = sum(len(set(ids32[q, :cnt32[q]].tolist()) & set(lo[0][q, :lo[2][q]].tolist())) for q in range(nq))
-        assert hits / max(1, int(cnt32.sum())) >= min_recall, (dtype, hits, int(cnt32.sum()))
+        print("dim %d dtype %d m %d: recall %.5f" % (dim, dtype, m, hits / max(1, int(cnt32.sum()))), flush=True)
+        assert hits / max(1, int(cnt32.sum())) >= 0.99, (dtype, m, hits, int(cnt32.sum()))
     t.free()
 
 
