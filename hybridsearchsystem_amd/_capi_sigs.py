@@ -28,6 +28,8 @@ class IndexPostingField(ctypes.Structure):
 
 def bind(L):
     L.frk_check_program.restype = i32
+    L.frk_mlp_forward_noalloc.restype = i32
+    L.frk_mlp_forward_noalloc.argtypes = [vp, i32, vp, i32, vp, vp]
     L.frk_check_program.argtypes = [vp, u32, u32, vp]
     L.frk_tiles_bytes.restype = u64
     L.frk_tiles_bytes.argtypes = [u64, u32, u32]

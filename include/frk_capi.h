@@ -253,6 +253,17 @@ typedef struct {
 int frk_mlp_forward(const frk_mlp_params* p, int32_t rows, const float* d_x, int32_t ldx, float* d_work, float* d_y,
                     frk_stream_t stream);
 
+/* frk_mlp_forward without a caller workspace, for callers that cannot allocate device memory (code under
+ * graph capture, tight serving loops, harnesses that forbid allocation); no counterpart in the
+ * reference. The four activation tiles live in a fixed device array of 4 x FRK_MLP_NOALLOC_MAX_ROWS x
+ * FRK_MLP_NOALLOC_MAX_HIDDEN floats (4 MiB, held by each GPU the library runs on until the process ends);
+ * larger calls fail with FRK_ERR_CAPACITY. Every call on one GPU shares that array, so calls that overlap
+ * on one GPU (different streams or threads) race; calls ordered on one stream do not. Allocates nothing. */
+#define FRK_MLP_NOALLOC_MAX_ROWS 1024
+#define FRK_MLP_NOALLOC_MAX_HIDDEN 256
+int frk_mlp_forward_noalloc(const frk_mlp_params* p, int32_t rows, const float* d_x, int32_t ldx, float* d_y,
+                            frk_stream_t stream);
+
 /* floors[r] = y[r] + offset[r] (an offset of -inf disables the floor of that row). */
 int frk_mlp_floors(int32_t rows, const float* d_y, const float* d_offset, float* d_floors, frk_stream_t stream);
 
