@@ -27,6 +27,10 @@ class IndexPostingField(ctypes.Structure):
 
 
 def bind(L):
+    L.frk_score_matrix.restype = i32
+    L.frk_score_matrix.argtypes = [vp, vp, u32, vp, u64, vp]
+    L.frk_aggregate.restype = i32
+    L.frk_aggregate.argtypes = [vp, vp, u64, vp, u32, vp, vp]
     L.frk_check_program.restype = i32
     L.frk_mlp_forward_noalloc.restype = i32
     L.frk_mlp_forward_noalloc.argtypes = [vp, i32, vp, i32, vp, vp]

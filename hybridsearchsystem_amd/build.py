@@ -1,8 +1,11 @@
-"""Builds the in-tree shared library ``hybridsearchsystem_amd/libfrk.so`` for gfx950.
+"""Builds the in-tree shared library ``hybridsearchsystem_amd/libfrk.so`` for gfx950, and the search
+driver ``hybridsearchsystem_amd/fr_search`` linked against it.
 
 Every ``csrc/*.hip`` file (device kernels and their C-ABI wrappers) and every ``csrc/**/*.cpp`` file
-(the C++ host engine and the handle API) is compiled with hipcc in parallel into ``_build/``, then
-linked. Run ``python -m hybridsearchsystem_amd.build`` (or ``__graft_entry__.build()``).
+outside ``csrc/driver`` (the C++ host engine and the handle API) is compiled with hipcc in parallel into
+``_build/``, then linked. The driver (``csrc/driver/*.cpp``) is compiled against the reference's include
+paths (``include/ref_paths``) and links ``libfrk.so`` with an rpath to its own directory. Run
+``python -m hybridsearchsystem_amd.build`` (or ``__graft_entry__.build()``).
 """
 import concurrent.futures as cf
 import glob
@@ -15,6 +18,8 @@ ROOT = os.path.dirname(HERE)
 CSRC = os.path.join(HERE, "csrc")
 BUILD = os.path.join(HERE, "_build")
 LIB = os.path.join(HERE, "libfrk.so")
+DRIVER = os.path.join(HERE, "fr_search")
+DRIVER_SRC = os.path.join(CSRC, "driver")
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 ARCH = "gfx950"
 
@@ -31,7 +36,8 @@ PER_FILE = {name: ["-mllvm", "-amdgpu-mfma-vgpr-form"]
 
 def _sources():
     hip = sorted(glob.glob(os.path.join(CSRC, "*.hip")))
-    cpp = sorted(glob.glob(os.path.join(CSRC, "**", "*.cpp"), recursive=True))
+    cpp = sorted(p for p in glob.glob(os.path.join(CSRC, "**", "*.cpp"), recursive=True)
+                 if not p.startswith(DRIVER_SRC + os.sep))
     return hip, cpp
 
 
@@ -74,7 +80,23 @@ def build(verbose=False, jobs=None):
     if p.returncode != 0:
         raise RuntimeError("link failed:\n" + p.stderr)
     os.replace(tmp, LIB)
+    _build_driver()
     return LIB
+
+
+def _build_driver():
+    """fr_search: the driver sources against the reference's include paths, linked to libfrk.so."""
+    srcs = sorted(glob.glob(os.path.join(DRIVER_SRC, "*.cpp")))
+    if not srcs:
+        return
+    tmp = DRIVER + ".tmp"
+    cmd = [HIPCC, "-O2", "-std=c++17", "-Wall", "--offload-arch=" + ARCH, "-I" + os.path.join(ROOT, "include", "ref_paths"),
+           "-I" + os.path.join(ROOT, "include"), "-o", tmp] + srcs + [
+        "-L" + HERE, "-lfrk", "-Wl,-rpath,$ORIGIN", "-lpthread"]
+    p = subprocess.run(cmd, capture_output=True, text=True)
+    if p.returncode != 0:
+        raise RuntimeError("fr_search build failed:\n" + p.stderr)
+    os.replace(tmp, DRIVER)
 
 
 if __name__ == "__main__":

@@ -447,4 +447,17 @@ double frk_server_now(const frk_server* s) { return s ? s->s->Now() : 0.0; }
 
 uint64_t frk_server_batch_bytes(const frk_server* s, uint32_t n_queries) { return s ? s->s->BatchBytes(n_queries) : 0; }
 
+int frk_workspace_pool_stats(frk_pool_stats* out) {
+    if (!out) return Fail(FRK_ERR_INVALID, "frk_workspace_pool_stats: null argument");
+    const WorkspacePoolStats st = GetWorkspacePoolStats();
+    out->live_bytes = st.liveBytes;
+    out->peak_bytes = st.peakBytes;
+    out->mallocs = st.mallocs;
+    out->frees = st.frees;
+    out->retain_limit = st.retainLimit;
+    out->workspaces = st.workspaces;
+    out->pooled = st.pooled;
+    return FRK_OK;
+}
+
 }  // extern "C"

@@ -86,6 +86,11 @@ bool PostingFieldData::FinishAdd(int32_t deviceId) {
         uint32_t head = 0;
         std::memcpy(&head, m_host.data() + p.offset, 4);
         v[p.segment] = (head & 0x0FFFFFFFu) ? base + p.offset : 0;
+        if (head & 0x0FFFFFFFu) {
+            uint32_t len = 0;
+            std::memcpy(&len, m_host.data() + p.offset + 4, 4);
+            m_describe[base + p.offset] = {static_cast<uint8_t>(head >> 28), len};
+        }
     }
     m_pending.clear();
     m_pending.shrink_to_fit();
@@ -103,6 +108,14 @@ const std::vector<uint64_t>* PostingFieldData::GetPostingDeviceAddrs(uint64_t to
     }
     tokenExist = true;
     return &it->second;
+}
+
+bool PostingFieldData::DescribePosting(uint64_t addr, uint8_t& layout, uint32_t& payloadBytes) const {
+    auto it = m_describe.find(addr);
+    if (it == m_describe.end()) return false;
+    layout = it->second.layout;
+    payloadBytes = it->second.payloadBytes;
+    return true;
 }
 
 const std::vector<uint8_t>* PostingFieldData::GetPostingTypes(uint64_t token) const {

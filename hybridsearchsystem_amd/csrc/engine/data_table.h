@@ -38,6 +38,9 @@ class PostingFieldData {
     // for an absent token (tokenExist = false), as posting_field_data.cpp:487-495.
     const std::vector<uint64_t>* GetPostingDeviceAddrs(uint64_t token, bool& tokenExist) const;
     const std::vector<uint8_t>* GetPostingTypes(uint64_t token) const;  // layout code per segment
+    // Layout and payload bytes of the posting whose head sits at device address addr (false for 0 and
+    // for addresses that are not posting heads of this field); for diagnostics.
+    bool DescribePosting(uint64_t addr, uint8_t& layout, uint32_t& payloadBytes) const;
     const std::string& Name() const { return m_name; }
     size_t TokenNum() const { return m_addrs.size(); }
     uint64_t ArenaBytes() const { return m_arenaBytes; }
@@ -58,6 +61,11 @@ class PostingFieldData {
     std::unordered_map<uint64_t, std::vector<uint64_t>> m_addrs;
     std::unordered_map<uint64_t, std::vector<uint8_t>> m_types;
     std::unordered_map<uint64_t, uint64_t> m_hits;
+    struct Described {
+        uint8_t layout;
+        uint32_t payloadBytes;
+    };
+    std::unordered_map<uint64_t, Described> m_describe;  // device head address -> posting shape
     std::vector<uint64_t> m_zeroAddrs;
     std::vector<uint8_t> m_zeroTypes;
     uint8_t* m_device = nullptr;
@@ -142,6 +150,12 @@ class DataTable {
     uint32_t GetSplitDocNumZn() const { return m_splitDocNumZn; }
     const std::unordered_set<std::string>& GetPostingFields() const { return m_postingFields; }
     int32_t GetDeviceId() const { return m_deviceId; }
+    // Layout and payload bytes of a posting head address of any posting field (diagnostics).
+    bool DescribePosting(uint64_t addr, uint8_t& layout, uint32_t& payloadBytes) const {
+        for (const auto& kv : m_postings)
+            if (kv.second->DescribePosting(addr, layout, payloadBytes)) return true;
+        return false;
+    }
     uint32_t GetVersion() const { return m_version; }
     std::string Describe() const;
 

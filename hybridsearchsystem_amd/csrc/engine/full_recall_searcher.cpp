@@ -5,11 +5,13 @@
 #include <chrono>
 #include <cmath>
 #include <limits>
-#include <cmath>
+#include <cstdio>
 #include <cstring>
 #include <mutex>
 
+#include "filter_stats.h"
 #include "log.h"
+#include "../frk_common.h"
 
 namespace NpuRetrieval {
 
@@ -17,19 +19,77 @@ namespace NpuRetrieval {
 // Device and pinned buffers reused across batches: grown to the high-water mark and never freed while
 // the process runs (the reference's GmMemoryManager pools, searcher/runtime/gm_memory_manager.cpp:9-75,
 // without per-call malloc/free). One workspace per concurrent searcher, each with its own stream.
+// Process-wide accounting of the workspace pool's device memory (the [POOLSTAT] report, and the
+// retention limit a server sets from its HBM budget).
+namespace {
+struct PoolCounters {
+    std::atomic<uint64_t> liveBytes{0}, peakBytes{0}, mallocs{0}, frees{0};
+    std::atomic<uint32_t> workspaces{0}, pooled{0};
+    std::atomic<uint64_t> retainLimit{0};  // 0 = keep every released workspace
+};
+PoolCounters& Counters() {
+    static PoolCounters c;
+    return c;
+}
+void NoteDeviceAlloc(uint64_t bytes) {
+    PoolCounters& c = Counters();
+    const uint64_t live = c.liveBytes.fetch_add(bytes) + bytes;
+    uint64_t peak = c.peakBytes.load();
+    while (live > peak && !c.peakBytes.compare_exchange_weak(peak, live)) {
+    }
+    c.mallocs++;
+}
+void NoteDeviceFree(uint64_t bytes) {
+    Counters().liveBytes.fetch_sub(bytes);
+    Counters().frees++;
+}
+}  // namespace
+
+WorkspacePoolStats GetWorkspacePoolStats() {
+    PoolCounters& c = Counters();
+    WorkspacePoolStats s;
+    s.liveBytes = c.liveBytes.load();
+    s.peakBytes = c.peakBytes.load();
+    s.mallocs = c.mallocs.load();
+    s.frees = c.frees.load();
+    s.workspaces = c.workspaces.load();
+    s.pooled = c.pooled.load();
+    s.retainLimit = c.retainLimit.load();
+    return s;
+}
+
+void ReportWorkspacePool(const char* why) {
+    const WorkspacePoolStats s = GetWorkspacePoolStats();
+    std::printf("[POOLSTAT] pool=search_workspace %s liveMB=%.1f peakMB=%.1f workspaces=%u pooled=%u mallocs=%llu "
+                "frees=%llu retainLimitMB=%.1f\n",
+                why, s.liveBytes / 1048576.0, s.peakBytes / 1048576.0, s.workspaces, s.pooled,
+                static_cast<unsigned long long>(s.mallocs), static_cast<unsigned long long>(s.frees),
+                s.retainLimit / 1048576.0);
+    std::fflush(stdout);
+}
+
 class SearchWorkspace {
    public:
     struct Buf {
         void* p = nullptr;
         size_t n = 0;
         bool pinned = false;
+        void Free() {
+            if (!p) return;
+            if (pinned) {
+                (void)hipHostFree(p);
+            } else {
+                (void)hipFree(p);
+                NoteDeviceFree(n);
+            }
+            p = nullptr;
+            n = 0;
+        }
         bool Ensure(size_t bytes) {
             if (bytes <= n) return true;
             size_t want = std::max(bytes, n + n / 4);
             want = (want + 4095) / 4096 * 4096;
-            if (p) pinned ? (void)hipHostFree(p) : (void)hipFree(p);
-            p = nullptr;
-            n = 0;
+            Free();
             hipError_t e = pinned ? hipHostMalloc(&p, want, hipHostMallocDefault) : hipMalloc(&p, want);
             if (e != hipSuccess) {
                 p = nullptr;
@@ -37,6 +97,7 @@ class SearchWorkspace {
                 return false;
             }
             n = want;
+            if (!pinned) NoteDeviceAlloc(want);
             return true;
         }
         template <typename T>
@@ -46,6 +107,7 @@ class SearchWorkspace {
     };
 
     explicit SearchWorkspace(int32_t device) : m_device(device) {
+        Counters().workspaces++;
         (void)hipSetDevice(device);
         (void)hipStreamCreateWithFlags(&stream, hipStreamNonBlocking);
         for (auto& e : ev) (void)hipEventCreate(&e);
@@ -55,9 +117,10 @@ class SearchWorkspace {
         (void)hipSetDevice(m_device);
         for (auto& g : m_graphs) (void)hipGraphExecDestroy(g.exec);
         for (Buf* b : {&batch, &bitmaps, &bmasks, &counters, &cand, &probe, &lp, &rescored, &out, &mlp, &hbatch, &hout})
-            if (b->p) b->pinned ? (void)hipHostFree(b->p) : (void)hipFree(b->p);
+            b->Free();
         for (auto& e : ev) (void)hipEventDestroy(e);
         (void)hipStreamDestroy(stream);
+        Counters().workspaces--;
     }
 
     // Instantiated pipeline graphs of this workspace, keyed by every scalar and address the captured
@@ -89,14 +152,48 @@ class SearchWorkspace {
             if ((*it)->m_device == device) {
                 auto w = std::move(*it);
                 pool.erase(it);
+                Counters().pooled--;
                 return w;
             }
         }
         return std::make_unique<SearchWorkspace>(device);
     }
+    // Back to the pool, unless the workspaces then hold more device memory than the retention limit:
+    // such a workspace is freed (its stream synchronised first), so idle workspaces never keep the
+    // process above the limit.
+    // Frees idle workspaces while the pool holds more than the retention limit.
+    static void TrimPool() {
+        std::vector<std::unique_ptr<SearchWorkspace>> drop;
+        {
+            std::lock_guard<std::mutex> g(PoolMutex());
+            const uint64_t limit = Counters().retainLimit.load();
+            auto& pool = Pool();
+            uint64_t live = Counters().liveBytes.load();
+            while (limit && live > limit && !pool.empty()) {
+                live -= std::min<uint64_t>(live, pool.back()->DeviceBytes());
+                drop.push_back(std::move(pool.back()));
+                pool.pop_back();
+                Counters().pooled--;
+            }
+        }
+        for (auto& w : drop) (void)hipStreamSynchronize(w->stream);
+    }
+    uint64_t DeviceBytes() const {
+        uint64_t n = 0;
+        for (const Buf* b : {&batch, &bitmaps, &bmasks, &counters, &cand, &probe, &lp, &rescored, &out, &mlp})
+            n += b->n;
+        return n;
+    }
     static void Release(std::unique_ptr<SearchWorkspace> w) {
+        const uint64_t limit = Counters().retainLimit.load();
+        if (limit && Counters().liveBytes.load() > limit) {
+            (void)hipStreamSynchronize(w->stream);
+            w.reset();
+            return;
+        }
         std::lock_guard<std::mutex> g(PoolMutex());
         Pool().push_back(std::move(w));
+        Counters().pooled++;
     }
 
     hipStream_t stream = nullptr;
@@ -120,6 +217,11 @@ class SearchWorkspace {
     std::vector<std::vector<uint64_t>> m_seen;
     int32_t m_device;
 };
+
+void SetWorkspaceRetainLimit(uint64_t bytes) {
+    Counters().retainLimit.store(bytes);
+    SearchWorkspace::TrimPool();
+}
 
 // ------------------------------------------------------------------------------------------ searcher
 FullRecallSearcher::FullRecallSearcher(int32_t deviceId, std::shared_ptr<DataTable> dataTable)
@@ -173,37 +275,205 @@ bool FullRecallSearcher::BatchSearchDevice(const std::string& vectorFieldName, b
     return RunDevice(*vf, deviceResults);
 }
 
+bool FullRecallSearcher::FillResult(const std::vector<uint64_t>& keys, bool isMultiShard, FullRecallResult& r) const {
+    const DocIdMapping* map = m_table->GetDocIdMapping();
+    r.isMultiShard = isMultiShard;
+    for (uint64_t key : keys) {
+        const uint32_t ldoc = frk::KeyDoc(key);
+        const float score = frk::KeyScore(key);
+        uint64_t gdoc = ldoc;
+        if (map && !map->GetGDocId(ldoc, gdoc)) {
+            FRK_LOG_ERROR("no global id for local doc " << ldoc);
+            return false;
+        }
+        if (isMultiShard) {
+            r.scoreWithIndexes.emplace_back(score, ldoc);
+            r.scoreWithIndexes.back().id = gdoc;
+        } else {
+            r.docIds.push_back(static_cast<int64_t>(gdoc));
+            r.scores.push_back(score);
+        }
+    }
+    return true;
+}
+
 bool FullRecallSearcher::BatchSearchExtract(std::vector<AggrDeviceResult>& dev,
                                             std::vector<std::shared_ptr<FullRecallResult>>& results,
                                             bool isMultiShard) {
-    const DocIdMapping* map = m_table->GetDocIdMapping();
     results.clear();
     results.reserve(dev.size());
     for (const AggrDeviceResult& d : dev) {
         auto r = std::make_shared<FullRecallResult>();
-        r->isMultiShard = isMultiShard;
-        for (uint64_t key : d.keys) {
-            const uint32_t ldoc = 0xFFFFFFFFu - static_cast<uint32_t>(key);
-            const uint32_t ord = static_cast<uint32_t>(key >> 32);
-            const uint32_t bits = (ord & 0x80000000u) ? (ord & 0x7FFFFFFFu) : ~ord;
-            float score;
-            std::memcpy(&score, &bits, 4);
-            uint64_t gdoc = ldoc;
-            if (map && !map->GetGDocId(ldoc, gdoc)) {
-                FRK_LOG_ERROR("no global id for local doc " << ldoc);
-                return false;
-            }
-            if (isMultiShard) {
-                r->scoreWithIndexes.emplace_back(score, ldoc);
-                r->scoreWithIndexes.back().id = gdoc;
-            } else {
-                r->docIds.push_back(static_cast<int64_t>(gdoc));
-                r->scores.push_back(score);
-            }
-        }
+        if (!FillResult(d.keys, isMultiShard, *r)) return false;
         results.push_back(std::move(r));
     }
     return true;
+}
+
+// ------------------------------------------------------------------------------------- unfused path
+namespace {
+void FreeBlock(GmBlock& b) {
+    if (b.data) (void)hipFree(b.data);
+    b.data = nullptr;
+    b.size = 0;
+}
+}  // namespace
+
+bool FullRecallSearcher::OneFilter(GmBlock& filterResultChunk) {
+    if (!m_table || m_trees.size() != 1) {
+        FRK_LOG_ERROR("OneFilter needs a data table and exactly one query (has " << m_trees.size() << ")");
+        return false;
+    }
+    if (hipSetDevice(m_deviceId) != hipSuccess) return false;
+    const uint32_t dnps = m_table->GetDocNumPerSegment(), segs = m_table->GetSegmentNum();
+    if (dnps % 64 != 0) {
+        FRK_LOG_ERROR("docNumPerSegment " << dnps << " is not a multiple of 64 (required by the device filter)");
+        return false;
+    }
+    FilterExpr fe;
+    if (!PrepareFilterExpr(*m_table, m_trees[0], fe)) return false;
+    frk_filter_query d{};
+    d.expr_len = fe.emptyTree ? 0 : static_cast<uint32_t>(fe.postExpr.size());
+    d.n_postings = fe.emptyTree ? 0 : static_cast<uint32_t>(fe.postingDeviceAddrs.size());
+    d.op_num = fe.opNum;
+    d.flags = fe.emptyTree ? FRK_FILTER_MATCH_ALL : 0;
+    std::vector<uint64_t> table;
+    for (uint32_t sg = 0; sg < segs && !fe.emptyTree; sg++)
+        for (const auto* addrs : fe.postingDeviceAddrs) table.push_back((*addrs)[sg]);
+    // one staging buffer: [descriptor | counters | program | operand table]
+    const size_t oExpr = 64, oTab = oExpr + (fe.postExpr.size() * 4 + 15) / 16 * 16;
+    std::vector<uint8_t> host(oTab + table.size() * 8 + 8, 0);
+    std::memcpy(host.data(), &d, sizeof(d));
+    if (!fe.postExpr.empty()) std::memcpy(host.data() + oExpr, fe.postExpr.data(), fe.postExpr.size() * 4);
+    if (!table.empty()) std::memcpy(host.data() + oTab, table.data(), table.size() * 8);
+    const size_t bytes = static_cast<size_t>(segs) * dnps / 8;
+    uint8_t* stage = nullptr;
+    char* out = nullptr;
+    if (hipMalloc(&stage, host.size()) != hipSuccess || hipMalloc(&out, bytes) != hipSuccess) {
+        (void)hipFree(stage);
+        FRK_LOG_ERROR("OneFilter: device allocation failed");
+        return false;
+    }
+    uint32_t cnt[2] = {0, 0};
+    bool ok = hipMemcpy(stage, host.data(), host.size(), hipMemcpyHostToDevice) == hipSuccess &&
+              frk_filter_eval(reinterpret_cast<const frk_filter_query*>(stage), 1,
+                              reinterpret_cast<const uint32_t*>(stage + oExpr),
+                              reinterpret_cast<const uint64_t*>(stage + oTab), segs, dnps, m_table->GetDocNum(),
+                              reinterpret_cast<uint64_t*>(out), bytes / 8, reinterpret_cast<uint32_t*>(stage + 32),
+                              reinterpret_cast<uint32_t*>(stage + 36), d.n_postings, nullptr) == FRK_OK &&
+              hipMemcpy(cnt, stage + 32, sizeof(cnt), hipMemcpyDeviceToHost) == hipSuccess;
+    (void)hipFree(stage);
+    if (!ok || cnt[1] != 0) {
+        FRK_LOG_ERROR("OneFilter: filter failed (" << frk_last_error() << ", status " << cnt[1] << ")");
+        (void)hipFree(out);
+        return false;
+    }
+    m_passing.assign(1, cnt[0]);
+    filterResultChunk.poolName = GmPoolName::TEXT_FILTER_RESULT_POOL;
+    filterResultChunk.data = out;
+    filterResultChunk.size = static_cast<uint32_t>(bytes);
+    return true;
+}
+
+bool FullRecallSearcher::BatchScore(const std::string& vectorFieldName, GmBlock& resultChunk) {
+    VectorFieldData* vf = nullptr;
+    if (!m_table || !m_table->GetVectorFieldData(vectorFieldName, vf)) {
+        FRK_LOG_ERROR("vector field not in the index: " << vectorFieldName);
+        return false;
+    }
+    const uint32_t Q = static_cast<uint32_t>(m_topKs.size());
+    if (Q == 0 || m_dim != vf->Dimension()) {
+        FRK_LOG_ERROR("BatchScore needs queries of the field's dimension " << vf->Dimension());
+        return false;
+    }
+    if (hipSetDevice(m_deviceId) != hipSuccess) return false;
+    const frk_doc_store* store = vf->Store(m_options.dtype);
+    if (!store) {
+        FRK_LOG_ERROR("vector store for dtype " << m_options.dtype << " was not built");
+        return false;
+    }
+    const uint32_t dp = store->dim;
+    const uint64_t stride = m_table->GetScoreExtendDocNum();
+    std::vector<float> q(static_cast<size_t>(Q) * dp, 0.0f);
+    for (uint32_t i = 0; i < Q; i++) std::memcpy(&q[static_cast<size_t>(i) * dp], &m_queries[static_cast<size_t>(i) * m_dim], 4 * m_dim);
+    float* dq = nullptr;
+    char* out = nullptr;
+    const size_t bytes = sizeof(float) * Q * stride;
+    if (bytes > 0xFFFFFFFFull) {
+        FRK_LOG_ERROR("BatchScore: a " << bytes << "-byte score matrix exceeds the block size field");
+        return false;
+    }
+    if (hipMalloc(&dq, q.size() * 4) != hipSuccess || hipMalloc(&out, bytes) != hipSuccess) {
+        (void)hipFree(dq);
+        FRK_LOG_ERROR("BatchScore: device allocation failed");
+        return false;
+    }
+    const bool ok = hipMemcpy(dq, q.data(), q.size() * 4, hipMemcpyHostToDevice) == hipSuccess &&
+                    frk_score_matrix(store, dq, Q, reinterpret_cast<float*>(out), stride, nullptr) == FRK_OK &&
+                    hipDeviceSynchronize() == hipSuccess;
+    (void)hipFree(dq);
+    if (!ok) {
+        FRK_LOG_ERROR("BatchScore: scoring failed: " << frk_last_error());
+        (void)hipFree(out);
+        return false;
+    }
+    resultChunk.poolName = GmPoolName::VECTOR_SCORE_RESULT_POOL;
+    resultChunk.data = out;
+    resultChunk.size = static_cast<uint32_t>(bytes);
+    resultChunk.readCount = std::make_shared<std::atomic<uint32_t>>(Q);
+    return true;
+}
+
+bool FullRecallSearcher::OneAggregator(GmBlock& resultChunk, GmBlock& filterResultChunk, uint32_t resultInDeviceOffset,
+                                       bool isMultiShard, std::vector<std::shared_ptr<FullRecallResult>>& results) {
+    if (!m_table || m_topKs.empty() || !resultChunk.data || !filterResultChunk.data) {
+        FRK_LOG_ERROR("OneAggregator needs a query, a score matrix and a filter block");
+        return false;
+    }
+    if (hipSetDevice(m_deviceId) != hipSuccess) return false;
+    const uint32_t k = m_topKs[0];
+    const uint32_t nDocs = m_table->GetDocNum();
+    const uint64_t stride = m_table->GetScoreExtendDocNum();
+    if (k > FRK_SELECT_MAX_K || (static_cast<uint64_t>(resultInDeviceOffset) + 1) * stride * 4 > resultChunk.size) {
+        FRK_LOG_ERROR("OneAggregator: k " << k << " or row " << resultInDeviceOffset << " out of range");
+        return false;
+    }
+    const float* row = reinterpret_cast<const float*>(resultChunk.data) + static_cast<uint64_t>(resultInDeviceOffset) * stride;
+    const uint32_t cap = std::max<uint32_t>(nDocs, 1);
+    uint64_t* cand = nullptr;
+    uint8_t* small = nullptr;  // [counter | out count | pad][k keys]
+    if (hipMalloc(&cand, sizeof(uint64_t) * cap) != hipSuccess ||
+        hipMalloc(&small, 256 + sizeof(uint64_t) * std::max<uint32_t>(k, 1)) != hipSuccess) {
+        (void)hipFree(cand);
+        FRK_LOG_ERROR("OneAggregator: device allocation failed");
+        return false;
+    }
+    uint32_t* cnt = reinterpret_cast<uint32_t*>(small);
+    uint64_t* outKeys = reinterpret_cast<uint64_t*>(small + 256);
+    std::vector<uint64_t> keys(k);
+    uint32_t counts[2] = {0, 0};
+    bool ok = hipMemset(small, 0, 256) == hipSuccess &&
+              frk_aggregate(row, reinterpret_cast<const uint64_t*>(filterResultChunk.data), nDocs, cand, cap, cnt,
+                            nullptr) == FRK_OK &&
+              frk_select_topk(cand, cap, cnt, 1, 1, nullptr, k, outKeys, std::max<uint32_t>(k, 1), cnt + 1, nullptr,
+                              nullptr) == FRK_OK &&
+              hipMemcpy(counts, cnt, sizeof(counts), hipMemcpyDeviceToHost) == hipSuccess &&
+              hipMemcpy(keys.data(), outKeys, sizeof(uint64_t) * k, hipMemcpyDeviceToHost) == hipSuccess;
+    (void)hipFree(cand);
+    (void)hipFree(small);
+    if (ok) {
+        keys.resize(std::min<uint32_t>(counts[1], k));
+        auto r = std::make_shared<FullRecallResult>();
+        ok = FillResult(keys, isMultiShard, *r);
+        if (ok) results.push_back(std::move(r));
+        m_passing.assign(1, counts[0]);
+    } else {
+        FRK_LOG_ERROR("OneAggregator: aggregation failed: " << frk_last_error());
+    }
+    // the score matrix is freed by its last reader, the filter block by its one reader
+    if (resultChunk.readCount && resultChunk.readCount->fetch_sub(1) == 1) FreeBlock(resultChunk);
+    FreeBlock(filterResultChunk);
+    return ok;
 }
 
 namespace {
@@ -257,9 +527,12 @@ bool FullRecallSearcher::RunDevice(const VectorFieldData& vf, std::vector<AggrDe
     std::vector<uint32_t> expr;
     std::vector<uint64_t> table;
     uint32_t maxPostings = 0;
+    const bool filterStats = FilterStatsEnabled() || EmptyStatsEnabled();
+    std::vector<FilterExpr> kept;  // the batch's compiled filters, kept only for the statistics
     for (uint32_t q = 0; q < Q; q++) {
         FilterExpr fe;
         if (!PrepareFilterExpr(*m_table, m_trees[q], fe)) return false;
+        if (filterStats) kept.push_back(fe);
         frk_filter_query& d = descs[q];
         d.expr_off = static_cast<uint32_t>(expr.size());
         d.table_off = static_cast<uint32_t>(table.size());
@@ -276,6 +549,11 @@ bool FullRecallSearcher::RunDevice(const VectorFieldData& vf, std::vector<AggrDe
     if (table.size() > 0xFFFFFFFFull || expr.size() > 0xFFFFFFFFull) {
         FRK_LOG_ERROR("filter tables too large for one batch");
         return false;
+    }
+    if (filterStats) {
+        std::vector<const FilterExpr*> ptrs;
+        for (const FilterExpr& fe : kept) ptrs.push_back(&fe);
+        NoteFilterBatch(*m_table, ptrs, segs, sizeof(uint64_t) * Q * maskStride);
     }
 
     // ---- pack the batch. Fixed-size regions first (descriptors | queries per store dim | k | k*m |

@@ -20,25 +20,10 @@
 #include <vector>
 
 #include "data_table.h"
+#include "full_recall_result.h"
 #include "query_node.h"
 
 namespace NpuRetrieval {
-
-struct ScoreWithIndex {
-    float score;
-    uint32_t index;  // local doc id
-    uint64_t id{0};  // global doc id
-    ScoreWithIndex(float s, uint32_t i) : score(s), index(i) {}
-};
-
-// Single shard: docIds (gdocid) and scores, best first, min(k, passing docs) long. Multi-shard:
-// scoreWithIndexes {score, ldocid, gdocid} in the same order (result_aggregator.cpp:846-1032).
-struct FullRecallResult {
-    bool isMultiShard = false;
-    std::vector<ScoreWithIndex> scoreWithIndexes;
-    std::vector<float> scores;
-    std::vector<int64_t> docIds;
-};
 
 // What BatchSearchDevice hands to BatchSearchExtract for one query: the sorted top-k keys.
 struct AggrDeviceResult {
@@ -72,6 +57,20 @@ struct BatchStats {
     float ms_host_finish = 0;    // host: from the device sync to the per-query results
 };
 
+// The device memory the searchers' workspace pool holds (grow-only per-batch buffers, one stream per
+// workspace; the reference's named GM pools, searcher/runtime/gm_memory_manager.cpp:9-75).
+struct WorkspacePoolStats {
+    uint64_t liveBytes = 0, peakBytes = 0, mallocs = 0, frees = 0, retainLimit = 0;
+    uint32_t workspaces = 0;  // alive (in use or pooled)
+    uint32_t pooled = 0;      // idle in the pool
+};
+WorkspacePoolStats GetWorkspacePoolStats();
+// A workspace released while the pool holds more than `bytes` of device memory is freed instead of
+// pooled (0 = no limit).
+void SetWorkspaceRetainLimit(uint64_t bytes);
+// One [POOLSTAT] line on stdout (gm_memory_pool.h:172-180 reports its pools the same way).
+void ReportWorkspacePool(const char* why);
+
 class SearchWorkspace;
 
 class FullRecallSearcher {
@@ -92,11 +91,24 @@ class FullRecallSearcher {
     bool BatchSearchExtract(std::vector<AggrDeviceResult>& deviceResults,
                             std::vector<std::shared_ptr<FullRecallResult>>& results, bool isMultiShard);
 
+    // The unfused path of the reference (full_recall_searcher.h:43-47): OneFilter evaluates the one
+    // query this searcher holds into a device bitmap block (segments x docNumPerSegment / 8 bytes);
+    // BatchScore writes the [queries][GetScoreExtendDocNum()] FP32 score matrix of every query added;
+    // OneAggregator compacts row resultInDeviceOffset of a score matrix by a filter block and selects
+    // the top-k of this searcher's first query, then frees the filter block and, after the last
+    // reader, the score matrix. The fused BatchSearch never takes this path.
+    bool OneFilter(GmBlock& filterResultChunk);
+    bool BatchScore(const std::string& vectorFieldName, GmBlock& resultChunk);
+    bool OneAggregator(GmBlock& resultChunk, GmBlock& filterResultChunk, uint32_t resultInDeviceOffset,
+                       bool isMultiShard, std::vector<std::shared_ptr<FullRecallResult>>& fullRecallResults);
+
     // Per-query filter pass counts of the last batch (for diagnostics and the learned bound).
     const std::vector<uint32_t>& LastPassing() const { return m_passing; }
 
    private:
     bool RunDevice(const VectorFieldData& vf, std::vector<AggrDeviceResult>& out);
+    // One query's keys (best first) as a result in either mode (result_aggregator.cpp:846-1032).
+    bool FillResult(const std::vector<uint64_t>& keys, bool isMultiShard, FullRecallResult& r) const;
 
     int32_t m_deviceId;
     std::shared_ptr<DataTable> m_table;

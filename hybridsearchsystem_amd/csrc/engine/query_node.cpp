@@ -8,6 +8,7 @@
 #include <mutex>
 
 #include "../../../include/frk_capi.h"
+#include "filter_stats.h"
 #include "log.h"
 
 namespace NpuRetrieval {
@@ -354,6 +355,7 @@ bool PrepareFilterExpr(const DataTable& table, QueryNode* tree, FilterExpr& out)
         FRK_LOG_ERROR("compiled filter program failed the structural check");
         return false;
     }
+    if (ExprStatsEnabled()) NoteExprShape(out.postExpr);
     if (PostExprMaxStack(out.postExpr) > FRK_FILTER_MAX_STACK ||
         out.postingDeviceAddrs.size() > FRK_FILTER_MAX_POSTINGS) {
         FRK_LOG_ERROR("filter needs a stack of " << PostExprMaxStack(out.postExpr) << " (max " << FRK_FILTER_MAX_STACK

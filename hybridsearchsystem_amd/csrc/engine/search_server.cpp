@@ -36,6 +36,9 @@ SearchServer::SearchServer(std::shared_ptr<DataTable> table, int32_t deviceId, c
     for (auto& t : m_inertTrees) t = std::make_unique<TermNode>(m_cfg.postingField, m_inertToken);
     m_cfg.workers = std::max<uint32_t>(1, m_cfg.workers);
     m_cfg.maxBatchQueries = std::max<uint32_t>(1, m_cfg.maxBatchQueries);
+    // workspaces are grow-only and pooled: without a retention limit, idle ones would keep their
+    // high-water memory on top of what the budget admits for the batches in flight
+    if (m_cfg.hbmBudgetBytes) SetWorkspaceRetainLimit(m_cfg.hbmBudgetBytes);
     m_slot.resize(m_cfg.workers);
     m_workerCv = std::vector<std::condition_variable>(m_cfg.workers);
     m_scheduler = std::thread([this] { SchedulerLoop(); });
@@ -53,6 +56,7 @@ SearchServer::~SearchServer() {
     if (m_scheduler.joinable()) m_scheduler.join();
     for (auto& t : m_workers)
         if (t.joinable()) t.join();
+    if (m_cfg.hbmBudgetBytes) SetWorkspaceRetainLimit(0);  // the limit was this server's
 }
 
 double SearchServer::Now() const {

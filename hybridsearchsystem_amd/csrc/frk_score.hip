@@ -97,11 +97,11 @@ int Prepare(const frk_doc_store* store, const float* d_queries, uint32_t n_queri
     return FRK_OK;
 }
 
-int Run(const frk_doc_store* store, const ScoreLaunch& L, bool block_max) {
+int Run(const frk_doc_store* store, const ScoreLaunch& L, int mode) {
     switch (store->dtype) {
-        case FRK_DTYPE_F32: return ScoreF32(L, block_max);
-        case FRK_DTYPE_BF16: return ScoreBf16(L, block_max);
-        case FRK_DTYPE_FP8E4M3: return ScoreFp8(L, block_max);
+        case FRK_DTYPE_F32: return ScoreF32(L, mode);
+        case FRK_DTYPE_BF16: return ScoreBf16(L, mode);
+        case FRK_DTYPE_FP8E4M3: return ScoreFp8(L, mode);
     }
     SetError("unknown dtype %u", store->dtype);
     return FRK_ERR_INVALID;
@@ -237,7 +237,7 @@ extern "C" int frk_score_select(const frk_doc_store* store, const float* d_queri
     int rc = frk::Prepare(store, d_queries, n_queries, d_block_masks, mask_q_stride, d_floor, block_begin,
                           block_step, block_count, d_cand, cand_cap, d_counts, counts_stride, stream,
                           "frk_score_select", L);
-    return rc ? rc : frk::Run(store, L, false);
+    return rc ? rc : frk::Run(store, L, frk::kModeSelect);
 }
 
 extern "C" int frk_score_blockmax(const frk_doc_store* store, const float* d_queries, uint32_t n_queries,
@@ -253,5 +253,21 @@ extern "C" int frk_score_blockmax(const frk_doc_store* store, const float* d_que
     int rc = frk::Prepare(store, d_queries, n_queries, d_block_masks, mask_q_stride, nullptr, block_begin,
                           block_step, block_count, reinterpret_cast<uint64_t*>(d_blockmax), stride, nullptr, 1,
                           stream, "frk_score_blockmax", L);
-    return rc ? rc : frk::Run(store, L, true);
+    return rc ? rc : frk::Run(store, L, frk::kModeBlockMax);
+}
+
+extern "C" int frk_score_matrix(const frk_doc_store* store, const float* d_queries, uint32_t n_queries, float* d_scores,
+                                uint64_t score_stride, frk_stream_t stream) {
+    if (!store || n_queries == 0 || store->n_docs == 0) return FRK_OK;
+    if (score_stride < store->n_docs || score_stride > 0xFFFFFFFFull) {
+        frk::SetError("frk_score_matrix: row stride %llu must hold %llu docs and fit 32 bits",
+                      (unsigned long long)score_stride, (unsigned long long)store->n_docs);
+        return FRK_ERR_CAPACITY;
+    }
+    const uint64_t n_blocks = (store->n_docs + 31) / 32;
+    frk::ScoreLaunch L;
+    int rc = frk::Prepare(store, d_queries, n_queries, nullptr, 0, nullptr, 0, 1, n_blocks,
+                          reinterpret_cast<uint64_t*>(d_scores), static_cast<uint32_t>(score_stride), nullptr, 1, stream,
+                          "frk_score_matrix", L);
+    return rc ? rc : frk::Run(store, L, frk::kModeDense);
 }

@@ -2,8 +2,11 @@
 #include "frk_score_impl.h"
 
 namespace frk {
-int ScoreBf16(const ScoreLaunch& L, bool block_max) {
-    return block_max ? Dispatch<FRK_DTYPE_BF16, true>(L.args, L.chunks, L.stream, L.grid_x)
-                     : Dispatch<FRK_DTYPE_BF16, false>(L.args, L.chunks, L.stream, L.grid_x);
+int ScoreBf16(const ScoreLaunch& L, int mode) {
+    switch (mode) {
+        case kModeBlockMax: return Dispatch<FRK_DTYPE_BF16, kModeBlockMax>(L.args, L.chunks, L.stream, L.grid_x);
+        case kModeDense: return Dispatch<FRK_DTYPE_BF16, kModeDense>(L.args, L.chunks, L.stream, L.grid_x);
+        default: return Dispatch<FRK_DTYPE_BF16, kModeSelect>(L.args, L.chunks, L.stream, L.grid_x);
+    }
 }
 }  // namespace frk

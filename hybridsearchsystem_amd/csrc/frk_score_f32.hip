@@ -2,8 +2,11 @@
 #include "frk_score_impl.h"
 
 namespace frk {
-int ScoreF32(const ScoreLaunch& L, bool block_max) {
-    return block_max ? Dispatch<FRK_DTYPE_F32, true>(L.args, L.chunks, L.stream, L.grid_x)
-                     : Dispatch<FRK_DTYPE_F32, false>(L.args, L.chunks, L.stream, L.grid_x);
+int ScoreF32(const ScoreLaunch& L, int mode) {
+    switch (mode) {
+        case kModeBlockMax: return Dispatch<FRK_DTYPE_F32, kModeBlockMax>(L.args, L.chunks, L.stream, L.grid_x);
+        case kModeDense: return Dispatch<FRK_DTYPE_F32, kModeDense>(L.args, L.chunks, L.stream, L.grid_x);
+        default: return Dispatch<FRK_DTYPE_F32, kModeSelect>(L.args, L.chunks, L.stream, L.grid_x);
+    }
 }
 }  // namespace frk

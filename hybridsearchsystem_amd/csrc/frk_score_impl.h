@@ -116,6 +116,12 @@ __device__ __forceinline__ void MfmaChunkPairFp8(const u32x4& a0, const u32x4& a
     acc = __builtin_amdgcn_mfma_scale_f32_32x32x64_f8f6f4(av, bv, acc, 0, 0, 0, sa, 0, sb);
 }
 
+// What a launch of the scorer produces: per-query candidate keys of the docs that pass filter and floor
+// (the fused pipeline), per (query, sampled block) maxima (the probe), or the whole score matrix.
+constexpr int kModeSelect = 0;
+constexpr int kModeBlockMax = 1;
+constexpr int kModeDense = 2;
+
 struct ScoreArgs {
     const u32x4* tiles;
     const float* doc_scale;
@@ -471,7 +477,26 @@ __device__ __forceinline__ void EpilogueBlockMax(const ScoreArgs& A, uint32_t q0
     }
 }
 
-template <int DT, int CH, int RB, bool kBlockMax>
+// Dense epilogue: every score of the block to the matrix, out[q * A.cap + doc] for docs below n_docs and
+// real queries. For each accumulator register, lanes 0-31 write 32 consecutive docs of one row and
+// lanes 32-63 those of the row four below: two 128-byte runs per store instruction.
+template <int RB>
+__device__ __forceinline__ void EpilogueDense(const ScoreArgs& A, uint32_t q0, uint64_t blk, const floatx16 (&acc)[RB]) {
+    const int lane = threadIdx.x & 63;
+    const uint64_t doc = blk * 32 + (lane & 31);
+    if (doc >= A.n_docs) return;
+    float* out = reinterpret_cast<float*>(A.cand);
+    const uint32_t half = lane >> 5;
+#pragma unroll
+    for (int rb = 0; rb < RB; rb++)
+#pragma unroll
+        for (int r = 0; r < 16; r++) {
+            const uint32_t q = q0 + 32 * rb + 8 * (r / 4) + 4 * half + (r % 4);
+            if (q < A.n_queries) out[static_cast<uint64_t>(q) * A.cap + doc] = acc[rb][r];
+        }
+}
+
+template <int DT, int CH, int RB, int kMode>
 __global__ __launch_bounds__((WgShape<DT, CH, RB>::kThreads), 1) void ScoreKernel(ScoreArgs A) {
     constexpr int kThreads = WgShape<DT, CH, RB>::kThreads;
     constexpr int kWaves = WgShape<DT, CH, RB>::kWaves;
@@ -599,8 +624,10 @@ __global__ __launch_bounds__((WgShape<DT, CH, RB>::kThreads), 1) void ScoreKerne
             for (int rb = 0; rb < RB; rb++) asm volatile("" : "+v"(acc[rb]));
             const uint64_t i = block_index(step);
             const uint64_t blk = A.block_begin + i * A.block_step;
-            if constexpr (kBlockMax)
+            if constexpr (kMode == kModeBlockMax)
                 EpilogueBlockMax<RB>(A, q0, i, acc);
+            else if constexpr (kMode == kModeDense)
+                EpilogueDense<RB>(A, q0, blk, acc);
             else
                 Epilogue<RB>(A, floor_lds, q0, blk, acc, Qu);
         }
@@ -615,7 +642,7 @@ __global__ __launch_bounds__((WgShape<DT, CH, RB>::kThreads), 1) void ScoreKerne
             if (base + decltype(s_c)::value < total) step_fn(s_c, base);
         }, std::make_integer_sequence<int, U - NG>{});
     }
-    if constexpr (!kBlockMax) FlushQueue(A, Qu);
+    if constexpr (kMode == kModeSelect) FlushQueue(A, Qu);
 }
 
 using KernelFn = void (*)(ScoreArgs);
@@ -623,13 +650,13 @@ using KernelFn = void (*)(ScoreArgs);
 // chunk counts per doc the scorer is compiled for, ascending (the cases of Dispatch below)
 constexpr uint32_t kScoreShapes[] = {4, 8, 12, 16, 24, 32, 48, 64, 96, 128};
 
-template <int DT, int CH, int RB, bool kBlockMax>
+template <int DT, int CH, int RB, int kMode>
 int Launch(const ScoreArgs& A, hipStream_t stream, uint32_t grid_x) {
     constexpr int kThreads = WgShape<DT, CH, RB>::kThreads;
     constexpr int kWaves = WgShape<DT, CH, RB>::kWaves;
     const size_t lds = static_cast<size_t>(RB) * CH * 64 * sizeof(u32x4) + 2 * 32 * RB * sizeof(float) +
                        kWaves * (kQueueKeys + kQueueRuns) * sizeof(uint64_t);
-    const void* fn = reinterpret_cast<const void*>(&ScoreKernel<DT, CH, RB, kBlockMax>);
+    const void* fn = reinterpret_cast<const void*>(&ScoreKernel<DT, CH, RB, kMode>);
     // workgroups of this shape one CU holds at once, with the LDS attribute set: once per process (a
     // function-local static is initialised exactly once even when several host threads launch together)
     struct Residency {
@@ -657,7 +684,7 @@ int Launch(const ScoreArgs& A, hipStream_t stream, uint32_t grid_x) {
     if (groups > 1) grid_x = grid_x >= kXcds ? grid_x / kXcds * kXcds : kXcds;
     ScoreArgs a = A;
     a.n_groups = groups;
-    hipLaunchKernelGGL((ScoreKernel<DT, CH, RB, kBlockMax>), dim3(grid_x * groups), dim3(kThreads), lds, stream, a);
+    hipLaunchKernelGGL((ScoreKernel<DT, CH, RB, kMode>), dim3(grid_x * groups), dim3(kThreads), lds, stream, a);
     FRK_HIP_CHECK(hipGetLastError());
     return FRK_OK;
 }
@@ -677,26 +704,28 @@ int Launch(const ScoreArgs& A, hipStream_t stream, uint32_t grid_x) {
 // every group re-reads the doc range (from the XCD's L2 after the first). Measured on 10M x 768 BF16
 // (with the staged query fragments): 192 queries 4.02 -> 3.53 ms, 128 queries (still two groups of 64)
 // 2.91 -> 2.75 ms from the staging alone.
-template <int DT, bool kBlockMax>
+template <int DT, int kMode>
 int Dispatch(const ScoreArgs& A, uint32_t ch, hipStream_t stream, uint32_t grid_x) {
     const uint32_t q = A.n_queries;
     int rb = q > 64 ? 4 : (q > 32 ? 2 : 1);
     if (rb > FRK_SCORE_MAX_RB) rb = FRK_SCORE_MAX_RB;
     // FP8 probe pass: two query groups of two row blocks (sharing an XCD's L2) measured faster
-    if (DT == FRK_DTYPE_FP8E4M3 && kBlockMax && rb > FRK_SCORE_FP8_PROBE_MAX_RB) rb = FRK_SCORE_FP8_PROBE_MAX_RB;
+    if (DT == FRK_DTYPE_FP8E4M3 && kMode == kModeBlockMax && rb > FRK_SCORE_FP8_PROBE_MAX_RB)
+        rb = FRK_SCORE_FP8_PROBE_MAX_RB;
+    if (kMode == kModeDense) rb = 1;  // the unfused score matrix is written one row block per workgroup
     const bool three = rb == 4 && (q + 95) / 96 < (q + 63) / 64;
 #define FRK_CASE(CHV)                                                                           \
     if (ch == CHV) {                                                                            \
-        if constexpr (CHV * 4 <= 128) {                                                         \
-            if (rb == 4) return Launch<DT, CHV, 4, kBlockMax>(A, stream, grid_x);               \
+        if constexpr (CHV * 4 <= 128 && kMode != kModeDense) {                                  \
+            if (rb == 4) return Launch<DT, CHV, 4, kMode>(A, stream, grid_x);                   \
         }                                                                                       \
-        if constexpr (CHV == 48 && !kBlockMax) {                                                \
-            if (three) return Launch<DT, CHV, 3, kBlockMax>(A, stream, grid_x);                 \
+        if constexpr (CHV == 48 && kMode == kModeSelect) {                                      \
+            if (three) return Launch<DT, CHV, 3, kMode>(A, stream, grid_x);                     \
         }                                                                                       \
-        if constexpr (CHV * 2 <= 128) {                                                         \
-            if (rb >= 2) return Launch<DT, CHV, 2, kBlockMax>(A, stream, grid_x);               \
+        if constexpr (CHV * 2 <= 128 && kMode != kModeDense) {                                  \
+            if (rb >= 2) return Launch<DT, CHV, 2, kMode>(A, stream, grid_x);                   \
         }                                                                                       \
-        return Launch<DT, CHV, 1, kBlockMax>(A, stream, grid_x);                                \
+        return Launch<DT, CHV, 1, kMode>(A, stream, grid_x);                                    \
     }
     FRK_CASE(4)
     FRK_CASE(8)
@@ -720,7 +749,7 @@ struct ScoreLaunch {
     uint32_t grid_x;
     hipStream_t stream;
 };
-int ScoreF32(const ScoreLaunch& L, bool block_max);
-int ScoreBf16(const ScoreLaunch& L, bool block_max);
-int ScoreFp8(const ScoreLaunch& L, bool block_max);
+int ScoreF32(const ScoreLaunch& L, int mode);
+int ScoreBf16(const ScoreLaunch& L, int mode);
+int ScoreFp8(const ScoreLaunch& L, int mode);
 }  // namespace frk

@@ -81,6 +81,16 @@ def score_select(store, d_queries, n_queries, d_block_masks, mask_q_stride, d_fl
                                  ptr(d_counts), counts_stride, stream_ptr(stream)), "frk_score_select")
 
 
+def score_matrix(store, d_queries, n_queries, d_scores, score_stride, stream=None):
+    check(lib().frk_score_matrix(ctypes.byref(store), ptr(d_queries), n_queries, ptr(d_scores), score_stride,
+                                 stream_ptr(stream)), "frk_score_matrix")
+
+
+def aggregate(d_scores, d_bits, n_docs, d_cand, cap, d_count, stream=None):
+    check(lib().frk_aggregate(ptr(d_scores), ptr(d_bits), n_docs, ptr(d_cand), cap, ptr(d_count), stream_ptr(stream)),
+          "frk_aggregate")
+
+
 def select_topk(d_cand, cand_cap, d_counts, n_queries, d_k, k_default, d_out, k_stride, d_out_counts, d_kth=None,
                 stream=None, counts_stride=1):
     check(lib().frk_select_topk(ptr(d_cand), cand_cap, ptr(d_counts), counts_stride, n_queries, ptr(d_k), k_default,

@@ -58,6 +58,8 @@ def _bind(L):
     L.frk_server_batch_bytes.argtypes = [vp, u32]
     L.frk_server_now.restype = ctypes.c_double
     L.frk_server_now.argtypes = [vp]
+    L.frk_workspace_pool_stats.restype = ctypes.c_int
+    L.frk_workspace_pool_stats.argtypes = [vp]
     L._frk_server_bound = True
     return L
 
@@ -120,6 +122,20 @@ class Server:
 
     def __del__(self):
         self.close()
+
+
+class PoolStats(ctypes.Structure):
+    """frk_pool_stats of include/frk_capi.h."""
+    _fields_ = [("live_bytes", ctypes.c_uint64), ("peak_bytes", ctypes.c_uint64), ("mallocs", ctypes.c_uint64),
+                ("frees", ctypes.c_uint64), ("retain_limit", ctypes.c_uint64), ("workspaces", ctypes.c_uint32),
+                ("pooled", ctypes.c_uint32)]
+
+
+def pool_stats():
+    """The searchers' workspace pool: device bytes held now and at the peak, workspaces alive/idle."""
+    st = PoolStats()
+    check(_bind(lib()).frk_workspace_pool_stats(ctypes.byref(st)), "frk_workspace_pool_stats")
+    return {f: getattr(st, f) for f, _ in PoolStats._fields_}
 
 
 def poisson_load(server, queries, filters, rate_qps, n_requests, max_request=256, ks=(10, 100, 1000), seed=0):

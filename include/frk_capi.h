@@ -4,7 +4,8 @@
  * Two layers, both plain C (no C++ or torch types):
  *
  *  1. Kernel entry points (frk_filter_eval, frk_masks_to_blocks, frk_score_select,
- *     frk_score_blockmax, frk_kth_largest, frk_select_topk, frk_rescore, frk_mlp_*). They replace the
+ *     frk_score_blockmax, frk_score_matrix, frk_aggregate, frk_kth_largest, frk_select_topk,
+ *     frk_rescore, frk_mlp_*). They replace the
  *     reference's five AscendC launch stubs, which the host stages call through
  *     ACLRT_LAUNCH_KERNEL (each entry cites the kernel it replaces). Stream-ordered, no implicit
  *     synchronisation, no allocation; the caller owns every device buffer. Reentrant across streams.
@@ -154,6 +155,23 @@ int frk_score_select(const frk_doc_store* store, const float* d_queries, uint32_
                      const uint32_t* d_block_masks, uint32_t mask_q_stride, const float* d_floor,
                      uint64_t block_begin, uint64_t block_step, uint64_t block_count, uint64_t* d_cand,
                      uint32_t cand_cap, uint32_t* d_counts, uint32_t counts_stride, frk_stream_t stream);
+
+/* The unfused score matrix, for callers that keep the reference's order of scoring every doc first and
+ * aggregating per query afterwards (FullRecallSearcher::BatchScore / OneAggregator): d_scores[q *
+ * score_stride + n] = <query q, doc n> for every n < n_docs (no filter; entries past n_docs untouched).
+ * Replaces kernel_vector_mmad in its unfused use (vector_scorer_mmad.cpp:127-134,194-198), which
+ * writes the [queries][extendDocNum] FP32 matrix; the same MFMA kernel as frk_score_select. */
+int frk_score_matrix(const frk_doc_store* store, const float* d_queries, uint32_t n_queries, float* d_scores,
+                     uint64_t score_stride, frk_stream_t stream);
+
+/* The unfused aggregator: for one query, every doc n < n_docs whose filter bit is set (d_bits: the
+ * words frk_filter_eval writes, bit n % 64 of word n / 64 for a table whose docs per segment are a
+ * multiple of 64) appends MakeKey(d_scores[n], n) to d_cand in no particular order; *d_count (zeroed by
+ * the caller) counts every passing doc, so a count above cap means the list was truncated. Replaces the
+ * Aggregator kernel (aggregator/kernel_aggregator.h:22-182, launched at result_aggregator.cpp:1090-1092);
+ * follow it with frk_select_topk. */
+int frk_aggregate(const float* d_scores, const uint64_t* d_bits, uint64_t n_docs, uint64_t* d_cand, uint32_t cap,
+                  uint32_t* d_count, frk_stream_t stream);
 
 /* Probe pass of the same kernel: per (query q, sampled block i) the largest score among the block's
  * filter-passing docs, or -inf, written to d_blockmax[q * stride + i] (no candidate lists, no
@@ -448,6 +466,17 @@ int frk_server_stats_get(const frk_server* s, frk_server_stats* out);
 /* The device working set the scheduler charges against the HBM budget for a batch of n queries. */
 uint64_t frk_server_batch_bytes(const frk_server* s, uint32_t n_queries);
 double frk_server_now(const frk_server* s);
+
+/* The searchers' workspace pool (grow-only per-batch device buffers, pooled between batches; the
+ * reference's named GM pools, searcher/runtime/gm_memory_manager.cpp:9-75, and their [POOLSTAT]
+ * report, gm_memory_pool.h:172-180): device bytes held now and at the peak, allocations, workspaces
+ * alive and idle, and the retention limit (a server with an HBM budget sets it to the budget: a
+ * workspace released while the pool holds more is freed instead of kept). */
+typedef struct {
+    uint64_t live_bytes, peak_bytes, mallocs, frees, retain_limit;
+    uint32_t workspaces, pooled;
+} frk_pool_stats;
+int frk_workspace_pool_stats(frk_pool_stats* out);
 
 #ifdef __cplusplus
 }

@@ -542,3 +542,56 @@ def test_blockmax_probe_and_kth_largest(frk):
         finite = (srt > -np.inf).sum()
         if ks[q] <= finite:
             assert (s64[bits[q], q] >= kth[q] - 1e-4).sum() >= ks[q]
+
+
+# ----------------------------------------------------------------------------- unfused score + aggregate
+@pytest.mark.parametrize("dtype", [0, 1, 2])
+def test_score_matrix_and_aggregate(frk, dtype):
+    """frk_score_matrix writes every (query, doc) score (FP32: within 1e-5 * sum|q_d v_d| of FP64;
+    BF16/FP8: the dot product of the stored, dequantized values) and leaves the row padding untouched;
+    frk_aggregate turns one row plus a filter bitmap into exactly the keys of the passing docs, and
+    counts past a short capacity."""
+    rng = np.random.default_rng(60 + dtype)
+    n, dim, nq = 5000 + 37, 96, 40
+    docs = rng.standard_normal((n, dim)).astype(np.float32)
+    qs = rng.standard_normal((nq, dim)).astype(np.float32)
+    tiles, scales, dim_pad = frk.tile_docs(docs, dtype)
+    store_t = dev(tiles)
+    scales_t = dev(scales) if scales is not None else None
+    store = frk.doc_store(store_t, scales_t, dtype, dim_pad, n)
+    qp = np.zeros((nq, dim_pad), np.float32)
+    qp[:, :dim] = qs
+    stride = (n + 31) // 32 * 32 + 64
+    print("score matrix dtype %d" % dtype, flush=True)
+    d_s = torch.full((nq, stride), 7.0, dtype=torch.float32, device="cuda")
+    frk.score_matrix(store, dev(qp), nq, d_s, stride)
+    torch.cuda.synchronize()
+    got = d_s.cpu().numpy()
+    assert np.all(got[:, n:] == 7.0)
+    if dtype == 0:
+        ref = docs.astype(np.float64) @ qs.astype(np.float64).T
+        scale = np.abs(docs.astype(np.float64)) @ np.abs(qs.astype(np.float64)).T
+        assert np.all(np.abs(got[:, :n].T - ref) <= 1e-5 * scale + 1e-30)
+    else:
+        # the dot product of the stored (dequantized) query and doc values
+        dd, dq = _dequantized(docs, dtype), _dequantized(qs, dtype)
+        ref = dd @ dq.T
+        scale = np.abs(dd) @ np.abs(dq).T
+        assert np.all(np.abs(got[:, :n].T - ref) <= 4e-5 * scale + 1e-30)
+    # aggregate row 3 by a random bitmap
+    bits = rng.random(n) < 0.3
+    words = oracle.words_from_bits(bits)
+    d_bits = dev(words.view(np.int64))
+    row = d_s[3].contiguous()
+    for cap in (n, 100):
+        d_cand = torch.zeros(cap, dtype=torch.int64, device="cuda")
+        d_cnt = torch.zeros(1, dtype=torch.int32, device="cuda")
+        frk.aggregate(row, d_bits, n, d_cand, cap, d_cnt)
+        torch.cuda.synchronize()
+        cnt = int(d_cnt.item())
+        assert cnt == int(bits.sum())
+        sc, docs_out = frk.decode_keys(d_cand.cpu().numpy().view(np.uint64)[:min(cnt, cap)])
+        if cap >= cnt:
+            assert np.array_equal(np.sort(docs_out), np.flatnonzero(bits))
+        assert bits[docs_out].all() and len(set(docs_out.tolist())) == len(docs_out)
+        assert np.array_equal(sc, got[3, docs_out])

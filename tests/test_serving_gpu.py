@@ -102,4 +102,13 @@ def test_server_result_same_alone_and_in_mix(corpus, budget_queries):
         assert a[5] == len(idx)
         assert np.array_equal(a[2], m[2]) and np.array_equal(a[0], m[0]) and np.array_equal(a[1], m[1])
         assert np.array_equal(a[3], m[3])
+    if budget_queries:
+        # idle workspaces are not kept beyond the budget: after the run (every batch done, every
+        # workspace released) the pool holds at most the budget
+        pool = serve.pool_stats()
+        print("pool after the budgeted run:", pool, "budget", budget, flush=True)
+        assert pool["retain_limit"] == budget
+        assert pool["live_bytes"] <= budget
     srv.close()
+    if budget_queries:
+        assert serve.pool_stats()["retain_limit"] == 0
