@@ -27,6 +27,8 @@ class IndexPostingField(ctypes.Structure):
 
 
 def bind(L):
+    L.frk_read_probe.restype = i32
+    L.frk_read_probe.argtypes = [vp, u64, vp, u32, vp]
     L.frk_score_matrix.restype = i32
     L.frk_score_matrix.argtypes = [vp, vp, u32, vp, u64, vp]
     L.frk_aggregate.restype = i32

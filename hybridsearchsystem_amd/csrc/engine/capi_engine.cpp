@@ -152,7 +152,7 @@ int frk_write_index(const char* dir, uint32_t version, uint32_t n_docs, uint32_t
     IndexWriter w(n_docs, docs_per_segment, version, split_doc_num_zn);
     for (uint32_t i = 0; i < n_vector_fields; i++) {
         const frk_index_vector_field& f = vector_fields[i];
-        if (!f.name || f.file_type > FRK_VECTOR_FILE_FP32_ROWS)
+        if (!f.name || f.file_type > FRK_VECTOR_FILE_FP8_ROWS)
             return Fail(FRK_ERR_INVALID, "frk_write_index: bad vector field");
         w.AddVectorField(f.name, f.dim, f.rows, static_cast<VectorFileType>(f.file_type));
     }

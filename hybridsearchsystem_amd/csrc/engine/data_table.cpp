@@ -1,5 +1,6 @@
 // data_table.cpp -- loading and building the device-resident index (see data_table.h).
 #include "data_table.h"
+#include "../frk_common.h"
 
 #include <hip/hip_runtime.h>
 
@@ -384,6 +385,29 @@ bool DataTable::LoadVectorField(const std::string& dataDir, const std::string& f
                 return false;
             }
             std::memcpy(rows.data(), body, n * dim * 4);
+        } else if (type == static_cast<uint16_t>(VectorFileType::BF16_ROWS)) {
+            if (bodyLen < n * dim * 2) {
+                FRK_LOG_ERROR("short BF16 vector body: " << path);
+                return false;
+            }
+            const uint16_t* h = reinterpret_cast<const uint16_t*>(body);
+            for (uint64_t i = 0; i < n * dim; i++) rows[i] = frk::Bf16ToF32(h[i]);
+        } else if (type == static_cast<uint16_t>(VectorFileType::FP8_ROWS)) {
+            // the stored values, scale x e4m3: a low-precision store built from them quantizes each
+            // back to the same value (the scale is a power of two and the E4M3 grid scales with it)
+            if (bodyLen < n * 4 + n * dim) {
+                FRK_LOG_ERROR("short FP8 vector body: " << path);
+                return false;
+            }
+            for (uint64_t i = 0; i < n; i++) {
+                float scale = 0.0f;
+                std::memcpy(&scale, body + 4 * i, 4);
+                const uint8_t* b = body + 4 * n + i * dim;
+                for (uint32_t d = 0; d < dim; d++) rows[i * dim + d] = scale * frk::E4M3ToF32(b[d]);
+            }
+        } else if (type != static_cast<uint16_t>(VectorFileType::FP16_ZN)) {
+            FRK_LOG_ERROR("unknown vector body type " << type << ": " << path);
+            return false;
         } else {
             // ZN FP16 blocks of Z docs; inside a block, dim block j of doc i at ((j * Z) + i) * 16
             // halves (indexer/builder/segment/vector_builder.cpp:46-66). The tail block may be cut short.

@@ -55,8 +55,10 @@ std::string IndexMetaToJson(const IndexMeta& meta);
 
 // Vector file header extension: the reference writes [u16 len=2][u16 dim] for its FP16 ZN body
 // (indexer/builder/segment/vector_builder.cpp:26-32). This build adds [u16 len=4][u16 dim][u16 dtype]
-// with dtype 1 = FP32 row-major body ([docs][dim] float).
-enum class VectorFileType : uint16_t { FP16_ZN = 0, FP32_ROWS = 1 };
+// with dtype 1 = FP32 row-major body ([docs][dim] float), 2 = BF16 rows ([docs][dim] bf16, nearest
+// even), 3 = FP8 E4M3 rows with a per-vector scale ([docs] f32 scales, the smallest power of two s with
+// max|v| / s <= 448, then [docs][dim] e4m3 bytes of v / s, nearest even, saturating).
+enum class VectorFileType : uint16_t { FP16_ZN = 0, FP32_ROWS = 1, BF16_ROWS = 2, FP8_ROWS = 3 };
 
 // Legacy FP16 codec of the reference's vector files: subnormals flush to zero both ways
 // (core/number/trans_number.cpp:84-105).

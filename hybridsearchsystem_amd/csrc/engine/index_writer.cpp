@@ -2,9 +2,12 @@
 #include "index_writer.h"
 
 #include <algorithm>
+#include <cmath>
 #include <cstring>
 #include <filesystem>
 #include <fstream>
+
+#include "../frk_common.h"
 
 namespace NpuRetrieval {
 
@@ -99,6 +102,25 @@ bool IndexWriter::WriteVectorSegment(const std::string& dir, const VectorField& 
     if (f.type == VectorFileType::FP32_ROWS) {
         Put<uint16_t>(ext, static_cast<uint16_t>(VectorFileType::FP32_ROWS));
         ok = WriteAll(path, EncodeFileHeader(static_cast<uint8_t>(m_version), ext), rows, n * f.dim * 4);
+    } else if (f.type == VectorFileType::BF16_ROWS) {
+        Put<uint16_t>(ext, static_cast<uint16_t>(VectorFileType::BF16_ROWS));
+        std::vector<uint16_t> body(n * f.dim);
+        for (uint64_t i = 0; i < n * f.dim; i++) body[i] = frk::F32ToBf16(rows[i]);
+        ok = WriteAll(path, EncodeFileHeader(static_cast<uint8_t>(m_version), ext), body.data(), body.size() * 2);
+    } else if (f.type == VectorFileType::FP8_ROWS) {
+        // per-vector power-of-two scales first, then the scaled E4M3 rows
+        Put<uint16_t>(ext, static_cast<uint16_t>(VectorFileType::FP8_ROWS));
+        std::vector<uint8_t> body(n * 4 + n * f.dim);
+        for (uint64_t i = 0; i < n; i++) {
+            const float* r = rows + i * f.dim;
+            float mx = 0.0f;
+            for (uint32_t d = 0; d < f.dim; d++) mx = std::max(mx, std::fabs(r[d]));
+            const float scale = frk::Fp8Scale(mx);
+            std::memcpy(body.data() + 4 * i, &scale, 4);
+            uint8_t* dst = body.data() + 4 * n + i * f.dim;
+            for (uint32_t d = 0; d < f.dim; d++) dst[d] = frk::F32ToE4M3(r[d] / scale);
+        }
+        ok = WriteAll(path, EncodeFileHeader(static_cast<uint8_t>(m_version), ext), body.data(), body.size());
     } else {
         // ZN: blocks of Z docs; inside a block, the 16-dim slice j of doc o sits at halves
         // ((b * Z) * J + Z * j + o) * 16, a partial last slice zero-filled (vector_builder.cpp:46-66).

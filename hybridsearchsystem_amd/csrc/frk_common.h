@@ -87,6 +87,18 @@ __host__ __device__ inline uint8_t F32ToE4M3(float f) {
     return static_cast<uint8_t>(sign | (be << 3) | m);
 }
 
+// BF16 bits -> FP32 (exact).
+__host__ __device__ inline float Bf16ToF32(uint16_t h) { return __builtin_bit_cast(float, static_cast<uint32_t>(h) << 16); }
+
+// OCP FP8 E4M3 (e4m3fn) bits -> FP32 (exact): bias 7, subnormals k * 2^-9, 0x7F / 0xFF NaN.
+__host__ __device__ inline float E4M3ToF32(uint8_t b) {
+    const float sign = (b & 0x80u) ? -1.0f : 1.0f;
+    const int e = (b >> 3) & 0xF, m = b & 0x7;
+    if (e == 15 && m == 7) return __builtin_nanf("");
+    if (e == 0) return sign * ldexpf(static_cast<float>(m), -9);
+    return sign * ldexpf(1.0f + m / 8.0f, e - 7);
+}
+
 // FP8 scale of a vector whose largest magnitude is mx: the smallest power of two s with mx / s <= 448
 // (E4M3's largest finite value), clamped to [2^-126, 2^127]; 1 for a zero vector. A power of two is an
 // E8M0 exponent, which the scaled FP8 MFMA applies per row and column by itself.

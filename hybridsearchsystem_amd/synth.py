@@ -17,8 +17,9 @@ CONFIGS = {
                ks=(100,), depth=3, tags_per_filter=(1, 4), sel=(0.001, 0.5), docs_per_segment=131072),
     "c2": dict(n_docs=5_000_000, dim=256, centroids=1024, n_tags=1024, densities="zipf", n_queries=4096, batch=64,
                ks=(10, 100, 1000), depth=3, tags_per_filter=(1, 4), sel=(0.0001, 0.6), docs_per_segment=131072),
-    "c3": dict(n_docs=10_000_000, dim=256, centroids=1024, n_tags=1024, densities="zipf", n_queries=1024, batch=128,
-               ks=(100,), depth=3, tags_per_filter=(1, 4), sel=(0.0001, 0.6), docs_per_segment=131072),
+    # the precision configuration takes the small configuration's tag and filter generator
+    "c3": dict(n_docs=10_000_000, dim=256, centroids=1024, n_tags=64, densities="loguniform", n_queries=1024,
+               batch=128, ks=(100,), depth=3, tags_per_filter=(1, 4), sel=(0.001, 0.5), docs_per_segment=131072),
 }
 
 SEEDS = dict(corpus=20261203, queries=1, filters=2, kdraw=3, tags=4, train_queries=11, train_filters=12,
@@ -177,6 +178,12 @@ def random_filters(seed, densities, n, tags_per_filter=(1, 4), depth=3, sel=(0.0
             out.append(_to_str(node))
             sels.append(s)
     return out, np.asarray(sels)
+
+
+def shift_tags(filters, offset):
+    """Filter strings with every tag id moved up by offset (two tag families in one posting field)."""
+    import re
+    return [None if f is None else re.sub(r"\d+", lambda m: str(int(m.group(0)) + offset), f) for f in filters]
 
 
 def draw_ks(seed, n, ks):

@@ -112,6 +112,11 @@ uint32_t frk_chunk_dims(uint32_t dtype);
  * chunks, the whole-chunk dim (such a store serves frk_rescore; frk_score_* refuse it). */
 uint32_t frk_store_dim(uint32_t dim, uint32_t dtype);
 
+/* Measurement aid, not on the search path: reads d_src[0 .. bytes) once with 16-byte loads from one
+ * resident wave of workgroups and writes one XOR word per workgroup to d_sink (at most sink_words
+ * workgroups), so the HBM read bandwidth this GPU attains can be timed beside a kernel's. */
+int frk_read_probe(const void* d_src, uint64_t bytes, uint32_t* d_sink, uint32_t sink_words, frk_stream_t stream);
+
 /* d[i] = value for i < n (a kernel, so it may sit inside a captured graph). */
 int frk_fill_u32(uint32_t* d, uint64_t n, uint32_t value, frk_stream_t stream);
 
@@ -333,10 +338,15 @@ int frk_builder_finish(frk_builder* b, frk_table** out); /* the builder stays ow
  * id/ *.idm, and per field one .vector or .posting file per segment -- the directory
  * frk_table_load reads. Host only. Vectors: rows [n_docs][dim] FP32; file_type
  * FRK_VECTOR_FILE_FP16_ZN is the reference's FP16 ZN body (split_doc_num_zn docs per block),
- * FRK_VECTOR_FILE_FP32_ROWS keeps FP32 rows. Postings: CSR over tokens, any id order, duplicates
- * dropped; token t's text is terms[t] when terms is non-null, else the decimal tags[t]. */
+ * FRK_VECTOR_FILE_FP32_ROWS keeps FP32 rows, FRK_VECTOR_FILE_BF16_ROWS BF16 rows (nearest even), and
+ * FRK_VECTOR_FILE_FP8_ROWS a per-vector FP32 scale (the smallest power of two s with max|v| / s <= 448)
+ * then E4M3 rows of v / s (nearest even, saturating); the loader widens BF16 / FP8 bodies to their
+ * stored values. Postings: CSR over tokens, any id order, duplicates dropped; token t's text is
+ * terms[t] when terms is non-null, else the decimal tags[t]. */
 #define FRK_VECTOR_FILE_FP16_ZN 0u
 #define FRK_VECTOR_FILE_FP32_ROWS 1u
+#define FRK_VECTOR_FILE_BF16_ROWS 2u
+#define FRK_VECTOR_FILE_FP8_ROWS 3u
 typedef struct frk_index_vector_field {
     const char* name;
     uint32_t dim;

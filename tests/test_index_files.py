@@ -160,6 +160,46 @@ def test_fp32_rows_and_string_terms(tmp_path):
             assert text.decode() in terms and h == oracle.hash64(text)
 
 
+def _fp8_scales(rows):
+    """Smallest power of two s with max|v| / s <= 448 per row (1 for an all-zero row)."""
+    mx = np.abs(rows).max(axis=1).astype(np.float64)
+    e = np.ceil(np.log2(np.where(mx > 0, mx, 448.0) / 448.0))
+    return np.where(mx > 0, 2.0 ** e, 1.0).astype(np.float32)
+
+
+def test_bf16_and_fp8_row_files(tmp_path):
+    """BF16 and FP8+scale vector bodies (header extension dtype 2 and 3): BF16 halves equal torch's
+    nearest-even BF16 rounding; FP8 files hold the per-vector power-of-two scales, then bytes equal to
+    torch's float8_e4m3fn rounding of v / s (saturating at 448)."""
+    torch = pytest.importorskip("torch")
+    rng = np.random.default_rng(21)
+    n, dim = 300, 40
+    vecs = (rng.standard_normal((n, dim)) * 10.0 ** rng.uniform(-3, 3, (n, 1))).astype(np.float32)
+    vecs[7] = 0.0
+    off, ids = _csr([np.arange(0, n, 3, dtype=np.uint32)])
+    s = _search()
+    for vf, code in ((s.VECTOR_FILE_BF16_ROWS, 2), (s.VECTOR_FILE_FP8_ROWS, 3)):
+        d = tmp_path / ("v%d" % code)
+        s.write_index(d, vecs, off, ids, docs_per_segment=128, vector_file=vf)
+        for seg in range(3):
+            first, cnt = seg * 128, min(128, n - seg * 128)
+            rows = vecs[first:first + cnt]
+            raw = (d / "vec" / ("poissonengine_0_%d_vec.vector" % seg)).read_bytes()
+            _, ext, hl = _header(raw)
+            assert struct.unpack("<HH", ext) == (dim, code)
+            body = raw[hl:]
+            if code == 2:
+                want = torch.from_numpy(rows).to(torch.bfloat16).view(torch.int16).numpy().view(np.uint16)
+                assert len(body) == cnt * dim * 2
+                assert np.array_equal(np.frombuffer(body, np.uint16).reshape(cnt, dim), want)
+            else:
+                sc = np.frombuffer(body[:4 * cnt], np.float32)
+                assert np.array_equal(sc, _fp8_scales(rows))
+                want = torch.from_numpy(rows / sc[:, None]).to(torch.float8_e4m3fn).view(torch.uint8).numpy()
+                assert len(body) == 4 * cnt + cnt * dim
+                assert np.array_equal(np.frombuffer(body[4 * cnt:], np.uint8).reshape(cnt, dim), want)
+
+
 def test_writer_rejects_bad_input(tmp_path):
     from hybridsearchsystem_amd import FrkError
     vecs, lists = _case(n=100, dim=8, n_tags=3)
@@ -176,7 +216,8 @@ def test_writer_rejects_bad_input(tmp_path):
 
 
 @pytest.mark.gpu
-@pytest.mark.parametrize("vector_file,is_matrix,packed", [(1, False, True), (0, False, False), (0, True, False)])
+@pytest.mark.parametrize("vector_file,is_matrix,packed", [(1, False, True), (0, False, False), (0, True, False),
+                                                        (2, False, True), (3, False, True)])
 def test_load_matches_in_memory_build(tmp_path, vector_file, is_matrix, packed):
     s = _search()
     n, dim, dps = 20000, 72, 4096
@@ -185,22 +226,29 @@ def test_load_matches_in_memory_build(tmp_path, vector_file, is_matrix, packed):
     lists = random_tags(rng, n, 24)
     off, ids = _csr(lists)
     gd = rng.permutation(n).astype(np.uint64) + 1000
+    import torch
     if vector_file == s.VECTOR_FILE_FP16_ZN:
         # the loader widens the stored halves: build in memory from the same widened values (the
         # encoder makes no subnormals, infinities or NaNs here, so IEEE widening is the loader's)
         mem_vecs = s.f32_to_f16(vecs).view(np.float16).astype(np.float32)
+    elif vector_file == s.VECTOR_FILE_BF16_ROWS:
+        mem_vecs = torch.from_numpy(vecs).to(torch.bfloat16).to(torch.float32).numpy()
+    elif vector_file == s.VECTOR_FILE_FP8_ROWS:
+        sc = _fp8_scales(vecs)[:, None]
+        mem_vecs = (torch.from_numpy(vecs / sc).to(torch.float8_e4m3fn).to(torch.float32).numpy() * sc).astype(np.float32)
     else:
         mem_vecs = vecs
     d = tmp_path / "idx"
     s.write_index(d, vecs, off, ids, docs_per_segment=dps, vector_file=vector_file, is_matrix=is_matrix,
                   packed=packed, gdocids=gd)
-    loaded = s.Table.load(str(d), dtypes=(s.FRK_DTYPE_BF16,))
-    built = s.Table.build(mem_vecs, off, ids, docs_per_segment=dps, dtypes=(s.FRK_DTYPE_BF16,), is_matrix=is_matrix,
-                          packed=packed, gdocids=gd)
+    lp = (s.FRK_DTYPE_BF16, s.FRK_DTYPE_FP8E4M3)
+    loaded = s.Table.load(str(d), dtypes=lp)
+    built = s.Table.build(mem_vecs, off, ids, docs_per_segment=dps, dtypes=lp, is_matrix=is_matrix, packed=packed,
+                          gdocids=gd)
     assert (loaded.n_docs, loaded.n_segments, loaded.docs_per_segment) == (n, 5, dps)
     q = rng.standard_normal((40, dim)).astype(np.float32)
     filters = ["%d" % (i % 24) if i % 4 else "%d AND NOT %d" % (i % 24, (i + 5) % 24) for i in range(39)] + [""]
-    for dtype in (s.FRK_DTYPE_F32, s.FRK_DTYPE_BF16):
+    for dtype in (s.FRK_DTYPE_F32, s.FRK_DTYPE_BF16, s.FRK_DTYPE_FP8E4M3):
         a = loaded.search(q, filters, 50, dtype=dtype)
         b = built.search(q, filters, 50, dtype=dtype)
         for x, y in zip(a[:4], b[:4]):
@@ -208,5 +256,40 @@ def test_load_matches_in_memory_build(tmp_path, vector_file, is_matrix, packed):
     # global ids come from the written id mapping
     ok = a[0] >= 0
     assert np.all(np.isin(a[0][ok], gd.astype(np.int64)))
+    loaded.free()
+    built.free()
+
+
+@pytest.mark.gpu
+def test_write_load_search_1m_docs_bf16_rows(tmp_path):
+    """1,048,576 docs x 128 in eight 131,072-doc segments written as BF16 rows, loaded with a BF16 store
+    and searched: plain FP32 results over the widened values equal an in-memory build's, and the BF16
+    pipeline (FP32 rescoring of the top k*4) matches them at recall >= 0.99."""
+    import torch
+    from hybridsearchsystem_amd import synth
+    s = _search()
+    n, dim, dps = 1 << 20, 128, 131072
+    print("write/load 1M x 128 BF16 rows", flush=True)
+    vecs = synth.mixture_vectors(31, n, dim, 64)
+    rng = np.random.default_rng(32)
+    dens = synth.loguniform_densities(rng, 32)
+    off, ids = synth.gen_postings(rng, n, dens)
+    d = tmp_path / "big"
+    s.write_index(d, vecs, off, ids, docs_per_segment=dps, vector_file=s.VECTOR_FILE_BF16_ROWS)
+    assert len(list((d / "vec").iterdir())) == 8
+    mem = torch.from_numpy(vecs).to(torch.bfloat16).to(torch.float32).numpy()
+    del vecs
+    loaded = s.Table.load(str(d), dtypes=(s.FRK_DTYPE_BF16,))
+    built = s.Table.build(mem, off, ids, docs_per_segment=dps, dtypes=(s.FRK_DTYPE_BF16,))
+    assert (loaded.n_docs, loaded.n_segments) == (n, 8)
+    qs = synth.mixture_queries(33, 64, dim, 64, corpus_seed=31)
+    fs, _ = synth.random_filters(34, dens, 64, depth=3, sel=(0.001, 0.5))
+    a = loaded.search(qs, fs, 100)
+    b = built.search(qs, fs, 100)
+    for x, y in zip(a[:4], b[:4]):
+        assert np.array_equal(x, y)
+    lo = loaded.search(qs, fs, 100, dtype=s.FRK_DTYPE_BF16, rescore_factor=4)
+    hits = sum(len(set(a[0][q, :a[2][q]].tolist()) & set(lo[0][q, :lo[2][q]].tolist())) for q in range(64))
+    assert hits / max(1, int(a[2].sum())) >= 0.99
     loaded.free()
     built.free()
