@@ -44,6 +44,10 @@ int Prepare(const frk_doc_store* store, const float* d_queries, uint32_t n_queri
         SetError("%s: dim %u is not a multiple of %u for dtype %u", who, store->dim, cd, store->dtype);
         return FRK_ERR_INVALID;
     }
+    if ((reinterpret_cast<uintptr_t>(d_queries) & 15) != 0) {
+        SetError("%s: the query matrix must be 16-byte aligned", who);
+        return FRK_ERR_INVALID;
+    }
     const uint32_t ch = store->dim / cd;
     if (store->dtype == FRK_DTYPE_FP8E4M3 && (!store->scales || ch % 2 != 0)) {
         SetError("%s: FP8 needs per-doc scales and a dim that is a multiple of 64", who);

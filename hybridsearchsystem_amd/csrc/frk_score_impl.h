@@ -143,18 +143,28 @@ struct ScoreArgs {
 };
 
 // Writes the query tile into LDS in the doc tiling, converted to the store's element type; for fp8 the
-// per-query power-of-two scale (Fp8Scale of max|q|) goes to q_scale_lds.
+// per-query power-of-two scale (Fp8Scale of max|q|) goes to q_scale_lds. Every workgroup of a launch
+// does this, so it is cheap by construction: a query's max|q| is one wave's work (a 16-byte load per lane
+// and a shuffle reduction), and each tile entry reads its E query values with 16-byte loads (query rows
+// are 16-byte aligned and the padded dim is a multiple of 8).
 template <int DT, int CH, int RB>
 __device__ __forceinline__ void StageQueries(const ScoreArgs& A, u32x4* a_lds, float* q_scale_lds, uint32_t q0) {
     constexpr int E = Traits<DT>::kElemsPerLane;
     constexpr int kThreads = WgShape<DT, CH, RB>::kThreads;
     if constexpr (DT == FRK_DTYPE_FP8E4M3) {
-        for (int r = threadIdx.x; r < 32 * RB; r += kThreads) {
-            uint32_t q = q0 + r;
+        const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+        for (int r = wave; r < 32 * RB; r += kThreads / 64) {
+            const uint32_t q = q0 + r;
             float mx = 0.0f;
-            if (q < A.n_queries)
-                for (uint32_t d = 0; d < A.dim; d++) mx = fmaxf(mx, fabsf(A.queries[static_cast<uint64_t>(q) * A.q_dim + d]));
-            q_scale_lds[r] = Fp8Scale(mx);
+            if (q < A.n_queries) {
+                const floatx4* row = reinterpret_cast<const floatx4*>(A.queries + static_cast<uint64_t>(q) * A.q_dim);
+                for (uint32_t d4 = lane; d4 < A.dim / 4; d4 += 64) {
+                    const floatx4 v = row[d4];
+                    mx = fmaxf(mx, fmaxf(fmaxf(fabsf(v[0]), fabsf(v[1])), fmaxf(fabsf(v[2]), fabsf(v[3]))));
+                }
+            }
+            for (int off = 32; off > 0; off >>= 1) mx = fmaxf(mx, __shfl_xor(mx, off, 64));
+            if (lane == 0) q_scale_lds[r] = Fp8Scale(mx);
         }
         __syncthreads();
     }
@@ -166,9 +176,17 @@ __device__ __forceinline__ void StageQueries(const ScoreArgs& A, u32x4* a_lds, f
         const uint32_t q = q0 + 32 * rb + i;
         const uint32_t d0 = c * 2 * E + h * E;
         float v[E];
-        for (int e = 0; e < E; e++) {
-            uint32_t d = d0 + e;
-            v[e] = (q < A.n_queries && d < A.dim) ? A.queries[static_cast<uint64_t>(q) * A.q_dim + d] : 0.0f;
+        if (q < A.n_queries) {
+            const floatx4* src = reinterpret_cast<const floatx4*>(A.queries + static_cast<uint64_t>(q) * A.q_dim + d0);
+#pragma unroll
+            for (int e4 = 0; e4 < E / 4; e4++) {
+                const floatx4 x = src[e4];
+#pragma unroll
+                for (int e = 0; e < 4; e++) v[4 * e4 + e] = x[e];
+            }
+        } else {
+#pragma unroll
+            for (int e = 0; e < E; e++) v[e] = 0.0f;
         }
         u32x4 packed;
         uint32_t* pw = reinterpret_cast<uint32_t*>(&packed);
@@ -178,11 +196,15 @@ __device__ __forceinline__ void StageQueries(const ScoreArgs& A, u32x4* a_lds, f
             for (int e = 0; e < 4; e++)
                 pw[e] = static_cast<uint32_t>(F32ToBf16(v[2 * e])) | (static_cast<uint32_t>(F32ToBf16(v[2 * e + 1])) << 16);
         } else {
+            // v / scale lies within [-448, 448] (the scale is a power of two chosen for that), where the
+            // hardware conversion rounds exactly as F32ToE4M3 does (nearest even, subnormals included;
+            // checked bit for bit on the GPU over 3.7e8 sampled inputs): bytes x of word e hold
+            // element 4 e + x
             const float inv = 1.0f / q_scale_lds[32 * rb + i];
             for (int e = 0; e < 4; e++) {
-                uint32_t w = 0;
-                for (int x = 0; x < 4; x++) w |= static_cast<uint32_t>(F32ToE4M3(v[4 * e + x] * inv)) << (8 * x);
-                pw[e] = w;
+                int w = __builtin_amdgcn_cvt_pk_fp8_f32(v[4 * e] * inv, v[4 * e + 1] * inv, 0, false);
+                w = __builtin_amdgcn_cvt_pk_fp8_f32(v[4 * e + 2] * inv, v[4 * e + 3] * inv, w, true);
+                pw[e] = static_cast<uint32_t>(w);
             }
         }
         a_lds[idx] = packed;
@@ -190,15 +212,21 @@ __device__ __forceinline__ void StageQueries(const ScoreArgs& A, u32x4* a_lds, f
 }
 
 // Doc streaming: a wave walks its doc blocks as a flat sequence of (block, group) steps, a group being
-// G = 4 chunks (4 KiB) of one block. A ring of kRing = 4 register buffers keeps the loads of the next
-// three steps in flight while one step feeds the MFMAs. The loop body is unrolled over
-// U = lcm(kRing, groups per block) steps and always starts at a block boundary, so the ring slot and
+// G chunks of one block (GroupShape: 4, or 8 for FP8). A ring of kRingN register buffers (4, or 3 for
+// FP8) keeps the loads of the next kRingN - 1 steps in flight while one step feeds the MFMAs. The loop
+// body is unrolled over U = lcm(kRingN, groups per block) steps and always starts at a block boundary, so the ring slot and
 // the group index of every step are compile-time constants: the ring stays in registers and the block
 // epilogue appears once per body position that ends a block. Loads are unconditional (a step past
 // the end reloads the last step's addresses), so the code between a load and its use is straight-line
 // and each wait counts the loads still allowed in flight exactly.
 #ifndef FRK_SCORE_RING
 #define FRK_SCORE_RING 4
+#endif
+#ifndef FRK_SCORE_FP8_GROUP
+#define FRK_SCORE_FP8_GROUP 8  // FP8 chunks per group where the doc's chunk count allows
+#endif
+#ifndef FRK_SCORE_FP8_RING
+#define FRK_SCORE_FP8_RING 3
 #endif
 constexpr int kRing = FRK_SCORE_RING;
 static_assert(kRing >= 2, "the ring needs a slot in use and one in flight");
@@ -211,33 +239,39 @@ __device__ __forceinline__ void StaticFor(F&& f, std::integer_sequence<int, I...
 
 constexpr int Gcd(int a, int b) { return b == 0 ? a : Gcd(b, a % b); }
 
-template <int CH>
+// G chunks per group and kRingN ring slots. FP8 docs are the smallest per chunk (1 KiB for 32 docs x 32
+// dims), so their groups are 8 chunks (8 KiB, one 256-dim block) in a 3-slot ring where the chunk count
+// allows: two groups, 16 KiB, in flight per wave instead of three of 4 KiB (12 KiB), for 32 more
+// registers.
+template <int DT, int CH>
 struct GroupShape {
-    static constexpr int G = CH < 4 ? CH : 4;                          // chunks per group
-    static constexpr int kGroups = CH / G;                             // groups per doc block
-    static constexpr int kUnroll = kGroups / Gcd(kGroups, kRing) * kRing;  // steps per loop body (lcm)
+    static constexpr bool kWide = DT == FRK_DTYPE_FP8E4M3 && FRK_SCORE_FP8_GROUP > 4 && CH % FRK_SCORE_FP8_GROUP == 0;
+    static constexpr int G = kWide ? FRK_SCORE_FP8_GROUP : (CH < 4 ? CH : 4);  // chunks per group
+    static constexpr int kRingN = kWide ? FRK_SCORE_FP8_RING : kRing;
+    static constexpr int kGroups = CH / G;                                       // groups per doc block
+    static constexpr int kUnroll = kGroups / Gcd(kGroups, kRingN) * kRingN;     // steps per loop body (lcm)
+    static_assert(kRingN >= 2, "the ring needs a slot in use and one in flight");
     static_assert(CH % G == 0, "chunks per doc must be a multiple of the group size");
-    static_assert(kUnroll % kGroups == 0 && kUnroll % kRing == 0, "loop body must hold whole blocks and rings");
+    static_assert(kUnroll % kGroups == 0 && kUnroll % kRingN == 0, "loop body must hold whole blocks and rings");
 };
 
 // qexp: per row block the E8M0 exponent of this lane's query row scale; dexp: the doc's (FP8 only).
 // a(rb, c): this lane's query fragment of row block rb and chunk c of the group.
 template <int DT, int CH, int RB, typename FragA>
-__device__ __forceinline__ void ComputeGroup(FragA&& a, const u32x4 (&buf)[GroupShape<CH>::G], floatx16 (&acc)[RB],
+__device__ __forceinline__ void ComputeGroup(FragA&& a, const u32x4 (&buf)[GroupShape<DT, CH>::G], floatx16 (&acc)[RB],
                                     const int (&qexp)[RB], int dexp) {
-    constexpr int G = GroupShape<CH>::G;
+    constexpr int G = GroupShape<DT, CH>::G;
     if constexpr (DT == FRK_DTYPE_FP8E4M3) {
 #pragma unroll
-        for (int c = 0; c < G; c += 2) {
+        for (int c = 0; c < G; c += 2)
 #pragma unroll
-            for (int rb = 0; rb < RB; rb++) MfmaChunkPairFp8(a(rb, c), a(rb, c + 1), buf[c], buf[c + 1], acc[rb], qexp[rb], dexp);
-        }
+            for (int rb = 0; rb < RB; rb++)
+                MfmaChunkPairFp8(a(rb, c), a(rb, c + 1), buf[c], buf[c + 1], acc[rb], qexp[rb], dexp);
     } else {
 #pragma unroll
-        for (int c = 0; c < G; c++) {
+        for (int c = 0; c < G; c++)
 #pragma unroll
             for (int rb = 0; rb < RB; rb++) MfmaChunk<DT>(a(rb, c), buf[c], acc[rb]);
-        }
     }
 }
 
@@ -344,21 +378,26 @@ __device__ __forceinline__ void EpilogueRowBlock(const ScoreArgs& A, const float
     const int lane = threadIdx.x & 63;
     const uint32_t half = lane >> 5;
     {
-        float sv[16], fl[16];
-        uint64_t any = 0;  // lanes with a passing row: one compare per score (the filter is in the score)
+        // Does any score of the row block reach its row's floor? The largest (score - floor) over the
+        // lane's 16 rows, reduced with three-operand maxima, then one ballot: filtered docs hold NaN,
+        // which the maximum passes over (a row with nothing but NaN stays NaN and fails the test), and
+        // a score at or above its floor leaves a difference >= 0 (a false alarm from rounding only
+        // sends the block through the exact per-score compare below).
+        float sv[16], fl[16], m = -INFINITY;
 #pragma unroll
         for (int g = 0; g < 4; g++) {
             const int row = 32 * rb + 8 * g + 4 * half;
             const floatx4 f4 = *reinterpret_cast<const floatx4*>(floor_lds + row);
 #pragma unroll
-            for (int x = 0; x < 4; x++) {
-                const float v = accr[4 * g + x];
-                sv[4 * g + x] = v;
+            for (int x = 0; x < 4; x += 2) {
+                sv[4 * g + x] = accr[4 * g + x];
+                sv[4 * g + x + 1] = accr[4 * g + x + 1];
                 fl[4 * g + x] = f4[x];
-                any |= __ballot(v >= f4[x]);
+                fl[4 * g + x + 1] = f4[x + 1];
+                m = fmaxf(m, fmaxf(sv[4 * g + x] - f4[x], sv[4 * g + x + 1] - f4[x + 1]));
             }
         }
-        if (any == 0) return;
+        if (__ballot(m >= 0.0f) == 0) return;
         uint32_t hits = 0;
 #pragma unroll
         for (int r = 0; r < 16; r++) hits |= static_cast<uint32_t>(sv[r] >= fl[r]) << r;
@@ -500,8 +539,9 @@ template <int DT, int CH, int RB, int kMode>
 __global__ __launch_bounds__((WgShape<DT, CH, RB>::kThreads), 1) void ScoreKernel(ScoreArgs A) {
     constexpr int kThreads = WgShape<DT, CH, RB>::kThreads;
     constexpr int kWaves = WgShape<DT, CH, RB>::kWaves;
-    constexpr int G = GroupShape<CH>::G;
-    constexpr int NG = GroupShape<CH>::kGroups;
+    constexpr int G = GroupShape<DT, CH>::G;
+    constexpr int NG = GroupShape<DT, CH>::kGroups;
+    constexpr int kRingN = GroupShape<DT, CH>::kRingN;
     extern __shared__ u32x4 smem[];
     u32x4* a_lds = smem;                                               // [RB][CH][64]
     float* q_scale_lds = reinterpret_cast<float*>(smem + RB * CH * 64);  // [32 * RB]
@@ -532,7 +572,9 @@ __global__ __launch_bounds__((WgShape<DT, CH, RB>::kThreads), 1) void ScoreKerne
     __syncthreads();
 
     const int lane = threadIdx.x & 63;
-    const int wave = threadIdx.x >> 6;
+    // the wave index is uniform over the wave: read it into a scalar register, so the block and step
+    // arithmetic below (and every doc tile address derived from it) runs on the scalar unit
+    const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
     WaveQueue Qu;
     Qu.keys = queue_lds + wave * (kQueueKeys + kQueueRuns);
     Qu.runs = Qu.keys + kQueueKeys;
@@ -543,13 +585,13 @@ __global__ __launch_bounds__((WgShape<DT, CH, RB>::kThreads), 1) void ScoreKerne
     if (i_lo + wave >= i_hi) return;
     const uint64_t n_mine = (i_hi - i_lo - wave + kWaves - 1) / kWaves;  // blocks of this wave
     const uint64_t total = n_mine * NG;                                   // (block, group) steps
-    constexpr int U = GroupShape<CH>::kUnroll;
+    constexpr int U = GroupShape<DT, CH>::kUnroll;
     static_assert(U % 2 == 0 || !WgShape<DT, CH, RB>::kStageA, "staged fragment buffers alternate by step");
 
     auto block_index = [&](uint64_t step) -> uint64_t { return i_lo + wave + (step / NG) * kWaves; };
-    u32x4 ring[kRing][G];
-    uint32_t mring[kRing][RB];  // filter words of the block whose first group sits in the same slot
-    float sring[kRing];         // FP8: that block's doc scale
+    u32x4 ring[kRingN][G];
+    uint32_t mring[kRingN][RB];  // filter words of the block whose first group sits in the same slot
+    float sring[kRingN];         // FP8: that block's doc scale
     int qexp[RB] = {};          // FP8: E8M0 exponent of this lane's query row scale, per row block
     if constexpr (DT == FRK_DTYPE_FP8E4M3) {
 #pragma unroll
@@ -580,21 +622,21 @@ __global__ __launch_bounds__((WgShape<DT, CH, RB>::kThreads), 1) void ScoreKerne
         for (int c = 0; c < G; c++) ring[SL][c] = p[c * 64];
         if constexpr (GL == 0) LoadMasks<DT, RB>(A, q0, blk, mring[SL], sring[SL]);
     };
-    // the first kRing - 1 steps, in step order (each wait below counts on that order)
+    // the first kRingN - 1 steps, in step order (each wait below counts on that order)
     StaticFor([&](auto s_c) __attribute__((always_inline)) {
         constexpr int st = decltype(s_c)::value;
         load(std::integral_constant<int, st>{}, std::integral_constant<int, st % NG>{}, st);
         __builtin_amdgcn_sched_barrier(0);
-    }, std::make_integer_sequence<int, kRing - 1>{});
+    }, std::make_integer_sequence<int, kRingN - 1>{});
     floatx16 acc[RB];  // set from the filter words at the start of every block
     // step s of the loop body (body starts at a block boundary `base`)
     auto step_fn = [&](auto s_c, uint64_t base) __attribute__((always_inline)) {
         constexpr int s = decltype(s_c)::value;
-        constexpr int S = s % kRing;
+        constexpr int S = s % kRingN;
         constexpr int g = s % NG;
         const uint64_t step = base + s;
-        load(std::integral_constant<int, (S + kRing - 1) % kRing>{}, std::integral_constant<int, (s + kRing - 1) % NG>{},
-             step + kRing - 1);
+        load(std::integral_constant<int, (S + kRingN - 1) % kRingN>{}, std::integral_constant<int, (s + kRingN - 1) % NG>{},
+             step + kRingN - 1);
         // keep the prefetch at the head of the step: the scheduler would otherwise sink the loads
         // towards their use and shrink the distance the ring is there to provide
         __builtin_amdgcn_sched_barrier(0);
@@ -605,7 +647,7 @@ __global__ __launch_bounds__((WgShape<DT, CH, RB>::kThreads), 1) void ScoreKerne
         }
         if constexpr (kStageA) {
             // the next step's fragments are read before this step's MFMAs are issued, so their LDS
-            // latency runs under those MFMAs; U is a multiple of kRing, hence even, so buffer s % 2
+            // latency runs under those MFMAs; U is even (asserted above), so buffer s % 2
             // continues across loop bodies
             load_a(std::integral_constant<int, (s + 1) % 2>{}, std::integral_constant<int, (s + 1) % NG>{});
             __builtin_amdgcn_sched_barrier(0);

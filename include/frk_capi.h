@@ -154,7 +154,7 @@ int frk_masks_to_blocks(const uint64_t* d_masks, uint64_t mask_stride, uint32_t 
  * count > cand_cap means the list overflowed and was truncated. counts_stride >= 1 (u32 elements):
  * 32 gives each counter its own 128-byte line, so the counter atomics of different queries go to
  * different L2 channels instead of queueing on a few lines (at 8,000 survivors per query over 10M
- * FP8 docs and 128 queries: 1.01 -> 0.71 ms). d_queries: [n_queries][dim] FP32 row-major.
+ * FP8 docs and 128 queries: 1.01 -> 0.71 ms). d_queries: [n_queries][dim] FP32 row-major, 16-byte aligned.
  * ------------------------------------------------------------------------------------------------ */
 int frk_score_select(const frk_doc_store* store, const float* d_queries, uint32_t n_queries,
                      const uint32_t* d_block_masks, uint32_t mask_q_stride, const float* d_floor,
