@@ -313,7 +313,7 @@ def precision_report(table, runs, docs, queries, filters, ks, args):
 
 def learned_leg(table, args, streams, res_p, centroids, seed, legs):
     """Trains the learned score floor on seeded queries drawn half from each filter generator (labels:
-    k'-th scores of the plain FP32 method, k' = k * m for m in {1, 4}), times each low-precision leg
+    k'-th scores of the plain FP32 method, k' = k * m for m in {1, 4, 8}), times each low-precision leg
     (dtype, m) with the learned floor and FP32 rescoring on the headline stream, and evaluates prune
     rate, fallback rate and exactness on held-out queries with k in {10, 100, 1000} and filters down to
     0.001% selectivity."""
@@ -333,7 +333,10 @@ def learned_leg(table, args, streams, res_p, centroids, seed, legs):
 
     tq, tf, tk = mixed(S["train_queries"], S["train_filters"], S["train_kdraw"], args.learned_train)
     log("learned floor: %d training queries drawn in %.1fs" % (args.learned_train, time.time() - t0))
-    model, info = learned.train_for_table(table, args.dim, tq, tf, tk, log=log)
+    # a query whose floor overshoots its k'-th score is rerun, and a rerun is a whole extra pass over
+    # the store, so the margin aims at one such query in 2,000 (about one batch of 128 in 16 pays it)
+    model, info = learned.train_for_table(table, args.dim, tq, tf, tk, rescore_factors=(1, 4, 8), val_fraction=0.15,
+                                          fallback=args.learned_fallback, log=log)
     log("learned floor: trained (%s)" % info)
     del tq
     runs, timed = [], {}
@@ -504,6 +507,8 @@ def main():
     ap.add_argument("--eager", action="store_true", help="launch every batch's kernels one by one (no hipGraph replay)")
     ap.add_argument("--learned-train", type=int, default=200_000, help="seeded training queries")
     ap.add_argument("--learned-eval", type=int, default=8192, help="held-out queries (k in {10,100,1000})")
+    ap.add_argument("--learned-fallback", type=float, default=0.0005,
+                    help="fraction of calibration queries whose learned floor may overshoot (each is rerun)")
     ap.add_argument("--traffic-profile", default=None,
                     help="rocprofv3 counter summary holding the main score kernel's HBM bytes per launch "
                          "(default: profiles/score_<dtype>_10Mx256_q128_traffic.md of the chosen pipeline)")

@@ -275,25 +275,21 @@ __device__ __forceinline__ void ComputeGroup(FragA&& a, const u32x4 (&buf)[Group
     }
 }
 
-// Survivor queue: a wave appends the (key) of every doc that passed its row's floor and filter bit to a
-// wave-private LDS queue, in runs of one (row, lane half) = one query each; the queue is flushed in bulk
-// (one counter atomic per run, all issued before any result is awaited, then the stores), so no global
+// Survivor queue: a wave appends each (score, doc) that passed its row's floor and filter bit, with the
+// row's index in the workgroup's query group, to a wave-private LDS queue; the queue is flushed in bulk
+// (one counter atomic per entry, all issued before any result is awaited, then the stores), so no global
 // memory access or atomic sits inside the MFMA loop.
 #ifndef FRK_SCORE_QUEUE_KEYS
-#define FRK_SCORE_QUEUE_KEYS 256  // 20 KiB of survivor queue per workgroup would leave room for only one per CU
+#define FRK_SCORE_QUEUE_KEYS 256  // entries per wave (9 bytes each)
 #endif
-constexpr int kQueueKeys = FRK_SCORE_QUEUE_KEYS;  // keys per wave
-constexpr int kQueueRuns = kQueueKeys / 4;        // runs per wave
-// accumulator rows appended per queue-capacity check: one row adds up to 64 keys (one per lane) and
-// 2 runs (one per lane half), so a group must fit an empty queue
-constexpr int kAppendRows = kQueueKeys / 64 < 16 ? kQueueKeys / 64 : 16;
-static_assert(kAppendRows >= 1 && 16 % kAppendRows == 0, "append groups must tile the 16 rows of a lane");
-static_assert(64 * kAppendRows <= kQueueKeys && 2 * kAppendRows <= kQueueRuns, "an append group must fit the queue");
+constexpr int kQueueKeys = FRK_SCORE_QUEUE_KEYS;
+constexpr int kQueueBytes = kQueueKeys * 9;  // per wave: kQueueKeys x (score, doc) u32 pairs, then the rows
+static_assert(kQueueKeys % 64 == 0 && kQueueKeys >= 64, "the queue flushes whole waves of entries");
 
 struct WaveQueue {
-    uint64_t* keys;  // [kQueueKeys]
-    uint64_t* runs;  // [kQueueRuns]: q | start << 32 | count << 48
-    uint32_t nk, nr; // uniform across the wave
+    uint2* ent;    // [kQueueKeys]: (score bits, doc)
+    uint8_t* row;  // [kQueueKeys]: accumulator row (query index within the workgroup's query group)
+    uint32_t n;    // uniform across the wave
 };
 
 __device__ __forceinline__ void WaveSync() {
@@ -301,30 +297,20 @@ __device__ __forceinline__ void WaveSync() {
     __builtin_amdgcn_wave_barrier();
 }
 
-__device__ __forceinline__ void FlushQueue(const ScoreArgs& A, WaveQueue& Qu) {
+__device__ __forceinline__ void FlushQueue(const ScoreArgs& A, uint32_t q0, WaveQueue& Qu) {
     const int lane = threadIdx.x & 63;
     WaveSync();
-    for (uint32_t i0 = 0; i0 < Qu.nr; i0 += 64) {
+    for (uint32_t i0 = 0; i0 < Qu.n; i0 += 64) {
         const uint32_t i = i0 + lane;
-        uint64_t run = 0;
-        uint32_t base = 0;
-        if (i < Qu.nr) {
-            run = Qu.runs[i];
-            base = atomicAdd(&A.counts[static_cast<uint64_t>(static_cast<uint32_t>(run)) * A.counts_stride],
-                             static_cast<uint32_t>(run >> 48));
-        }
-        if (i < Qu.nr) {
-            const uint32_t q = static_cast<uint32_t>(run);
-            const uint32_t start = static_cast<uint32_t>(run >> 32) & 0xFFFFu;
-            const uint32_t cnt = static_cast<uint32_t>(run >> 48);
-            uint64_t* dst = A.cand + static_cast<uint64_t>(q) * A.cap;
-            for (uint32_t e = 0; e < cnt; e++)
-                if (base + e < A.cap) dst[base + e] = Qu.keys[start + e];
+        if (i < Qu.n) {
+            const uint2 e = Qu.ent[i];
+            const uint32_t q = q0 + Qu.row[i];
+            const uint32_t at = atomicAdd(&A.counts[static_cast<uint64_t>(q) * A.counts_stride], 1u);
+            if (at < A.cap) A.cand[static_cast<uint64_t>(q) * A.cap + at] = MakeKey(__uint_as_float(e.x), e.y);
         }
     }
     WaveSync();
-    Qu.nk = 0;
-    Qu.nr = 0;
+    Qu.n = 0;
 }
 
 // The block's filter words for this wave's row blocks: for row block rb, lane l loads the word of doc
@@ -370,6 +356,12 @@ __device__ __forceinline__ void InitBias(const uint32_t (&m)[RB], bool doc_ok, f
     }
 }
 
+
+// Accumulator row r of a lane in lane half `half` holds query row 8 (r / 4) + 4 half + r % 4 of its row block.
+__device__ __forceinline__ uint32_t AccRow(int rb, int r, uint32_t half) {
+    return 32u * rb + 8u * (r / 4) + 4u * half + (r % 4);
+}
+
 // One row block of the epilogue; rb is a template parameter so the accumulator is always indexed
 // statically (a runtime row-block index would send it to scratch).
 template <int RB, int rb>
@@ -377,64 +369,43 @@ __device__ __forceinline__ void EpilogueRowBlock(const ScoreArgs& A, const float
                                         const floatx16& accr, WaveQueue& Qu) {
     const int lane = threadIdx.x & 63;
     const uint32_t half = lane >> 5;
-    {
-        // Does any score of the row block reach its row's floor? The largest (score - floor) over the
-        // lane's 16 rows, reduced with three-operand maxima, then one ballot: filtered docs hold NaN,
-        // which the maximum passes over (a row with nothing but NaN stays NaN and fails the test), and
-        // a score at or above its floor leaves a difference >= 0 (a false alarm from rounding only
-        // sends the block through the exact per-score compare below).
-        float sv[16], fl[16], m = -INFINITY;
+    // Does any score of the row block reach its row's floor? The largest (score - floor) over the lane's
+    // 16 rows, reduced with three-operand maxima, then one ballot: filtered docs hold NaN, which the
+    // maximum passes over (a row with nothing but NaN stays NaN and fails the test), and a score at or
+    // above its floor leaves a difference >= 0 (a false alarm from rounding only sends the block through
+    // the exact per-score compare below).
+    float mg[4];  // per group of four rows
 #pragma unroll
-        for (int g = 0; g < 4; g++) {
-            const int row = 32 * rb + 8 * g + 4 * half;
-            const floatx4 f4 = *reinterpret_cast<const floatx4*>(floor_lds + row);
+    for (int g = 0; g < 4; g++) {
+        const floatx4 f4 = *reinterpret_cast<const floatx4*>(floor_lds + 32 * rb + 8 * g + 4 * half);
+        mg[g] = fmaxf(fmaxf(accr[4 * g] - f4[0], accr[4 * g + 1] - f4[1]),
+                      fmaxf(accr[4 * g + 2] - f4[2], accr[4 * g + 3] - f4[3]));
+    }
+    if (__ballot(fmaxf(fmaxf(mg[0], mg[1]), fmaxf(mg[2], mg[3])) >= 0.0f) == 0) return;
+    // Survivors, four rows at a time: one room check per group for at most 4 per lane that holds any
+    // (never more than the queue), then the exact per-score compares with the group's floors read again
+    // (keeping them in registers across the block would cost 16 registers per row block).
+    const uint32_t d32 = static_cast<uint32_t>(doc);
 #pragma unroll
-            for (int x = 0; x < 4; x += 2) {
-                sv[4 * g + x] = accr[4 * g + x];
-                sv[4 * g + x + 1] = accr[4 * g + x + 1];
-                fl[4 * g + x] = f4[x];
-                fl[4 * g + x + 1] = f4[x + 1];
-                m = fmaxf(m, fmaxf(sv[4 * g + x] - f4[x], sv[4 * g + x + 1] - f4[x + 1]));
+    for (int g = 0; g < 4; g++) {
+        const uint64_t any = __ballot(mg[g] >= 0.0f);
+        if (any == 0) continue;
+        if (Qu.n + 4u * static_cast<uint32_t>(__popcll(any)) > static_cast<uint32_t>(kQueueKeys)) FlushQueue(A, q0, Qu);
+        const floatx4 f4 = *reinterpret_cast<const floatx4*>(floor_lds + 32 * rb + 8 * g + 4 * half);
+#pragma unroll
+        for (int x = 0; x < 4; x++) {
+            const int r = 4 * g + x;
+            const bool hit = accr[r] >= f4[x];
+            const uint64_t bal = __ballot(hit);
+            if (bal == 0) continue;
+            if (hit) {
+                const uint32_t below = __builtin_amdgcn_mbcnt_hi(static_cast<uint32_t>(bal >> 32),
+                                                                 __builtin_amdgcn_mbcnt_lo(static_cast<uint32_t>(bal), 0u));
+                const uint32_t slot = Qu.n + below;
+                Qu.ent[slot] = make_uint2(__float_as_uint(accr[r]), d32);
+                Qu.row[slot] = static_cast<uint8_t>(AccRow(rb, r, half));
             }
-        }
-        if (__ballot(m >= 0.0f) == 0) return;
-        uint32_t hits = 0;
-#pragma unroll
-        for (int r = 0; r < 16; r++) hits |= static_cast<uint32_t>(sv[r] >= fl[r]) << r;
-        // groups of kAppendRows rows: count a group's keys and runs first, flush if they would not fit,
-        // then append (so the unrolled append loop itself never flushes). A group holds at most
-        // 64 kAppendRows keys and 2 kAppendRows runs, which an empty queue always takes.
-#pragma unroll
-        for (int hb = 0; hb < 16 / kAppendRows; hb++) {
-            uint64_t bal[kAppendRows];
-            uint32_t nkeys = 0, nruns = 0;
-#pragma unroll
-            for (int x = 0; x < kAppendRows; x++) {
-                bal[x] = __ballot((hits >> (kAppendRows * hb + x)) & 1u);
-                nkeys += static_cast<uint32_t>(__popcll(bal[x]));
-                nruns += (static_cast<uint32_t>(bal[x]) != 0) + (static_cast<uint32_t>(bal[x] >> 32) != 0);
-            }
-            if (nkeys == 0) continue;
-            if (Qu.nk + nkeys > kQueueKeys || Qu.nr + nruns > kQueueRuns) FlushQueue(A, Qu);
-#pragma unroll
-            for (int x = 0; x < kAppendRows; x++) {
-                const int r = kAppendRows * hb + x;
-                if (bal[x] == 0) continue;
-                const uint32_t lo = static_cast<uint32_t>(bal[x]), hi = static_cast<uint32_t>(bal[x] >> 32);
-                const uint32_t below = static_cast<uint32_t>(__popcll(bal[x] & ((1ull << lane) - 1ull)));
-                if ((hits >> r) & 1u) Qu.keys[Qu.nk + below] = MakeKey(sv[r], static_cast<uint32_t>(doc));
-                const uint32_t qa = q0 + 32 * rb + 8 * (r / 4) + (r % 4);  // row of lane half 0
-                const uint32_t nlo = static_cast<uint32_t>(__popc(lo)), nhi = static_cast<uint32_t>(__popc(hi));
-                if (lane == 0) {
-                    uint32_t at = Qu.nr;
-                    if (nlo) Qu.runs[at++] = qa | (static_cast<uint64_t>(Qu.nk) << 32) | (static_cast<uint64_t>(nlo) << 48);
-                    if (nhi)
-                        Qu.runs[at] =
-                            (qa + 4) | (static_cast<uint64_t>(Qu.nk + nlo) << 32) | (static_cast<uint64_t>(nhi) << 48);
-                }
-                Qu.nk += nlo + nhi;
-                Qu.nr += (nlo != 0) + (nhi != 0);
-            }
+            Qu.n += static_cast<uint32_t>(__popcll(bal));
         }
     }
 }
@@ -546,7 +517,7 @@ __global__ __launch_bounds__((WgShape<DT, CH, RB>::kThreads), 1) void ScoreKerne
     u32x4* a_lds = smem;                                               // [RB][CH][64]
     float* q_scale_lds = reinterpret_cast<float*>(smem + RB * CH * 64);  // [32 * RB]
     float* floor_lds = q_scale_lds + 32 * RB;                          // [32 * RB]
-    uint64_t* queue_lds = reinterpret_cast<uint64_t*>(floor_lds + 32 * RB);  // [waves][keys + runs]
+    uint8_t* queue_lds = reinterpret_cast<uint8_t*>(floor_lds + 32 * RB);  // [waves][kQueueBytes]
     // Workgroup -> (doc range x, query group): with several query groups every doc range is read once
     // per group, so the groups of one range go to the same XCD (workgroups are dealt to the 8 XCDs
     // round-robin by linear id) and run back to back there, sharing that XCD's L2 instead of each
@@ -576,9 +547,9 @@ __global__ __launch_bounds__((WgShape<DT, CH, RB>::kThreads), 1) void ScoreKerne
     // arithmetic below (and every doc tile address derived from it) runs on the scalar unit
     const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
     WaveQueue Qu;
-    Qu.keys = queue_lds + wave * (kQueueKeys + kQueueRuns);
-    Qu.runs = Qu.keys + kQueueKeys;
-    Qu.nk = Qu.nr = 0;
+    Qu.ent = reinterpret_cast<uint2*>(queue_lds + wave * kQueueBytes);
+    Qu.row = queue_lds + wave * kQueueBytes + kQueueKeys * sizeof(uint2);
+    Qu.n = 0;
     const uint64_t per_wg = (A.block_count + n_ranges - 1) / n_ranges;
     const uint64_t i_lo = static_cast<uint64_t>(range_x) * per_wg;
     const uint64_t i_hi = i_lo + per_wg < A.block_count ? i_lo + per_wg : A.block_count;
@@ -684,7 +655,7 @@ __global__ __launch_bounds__((WgShape<DT, CH, RB>::kThreads), 1) void ScoreKerne
             if (base + decltype(s_c)::value < total) step_fn(s_c, base);
         }, std::make_integer_sequence<int, U - NG>{});
     }
-    if constexpr (kMode == kModeSelect) FlushQueue(A, Qu);
+    if constexpr (kMode == kModeSelect) FlushQueue(A, q0, Qu);
 }
 
 using KernelFn = void (*)(ScoreArgs);
@@ -697,7 +668,7 @@ int Launch(const ScoreArgs& A, hipStream_t stream, uint32_t grid_x) {
     constexpr int kThreads = WgShape<DT, CH, RB>::kThreads;
     constexpr int kWaves = WgShape<DT, CH, RB>::kWaves;
     const size_t lds = static_cast<size_t>(RB) * CH * 64 * sizeof(u32x4) + 2 * 32 * RB * sizeof(float) +
-                       kWaves * (kQueueKeys + kQueueRuns) * sizeof(uint64_t);
+                       kWaves * kQueueBytes;
     const void* fn = reinterpret_cast<const void*>(&ScoreKernel<DT, CH, RB, kMode>);
     // workgroups of this shape one CU holds at once, with the LDS attribute set: once per process (a
     // function-local static is initialised exactly once even when several host threads launch together)
