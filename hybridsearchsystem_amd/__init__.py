@@ -11,7 +11,7 @@ import os
 HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.environ.get("FRK_LIB_PATH") or os.path.join(HERE, "libfrk.so")
 
-FRK_DTYPE_F32, FRK_DTYPE_BF16, FRK_DTYPE_FP8E4M3 = 0, 1, 2
+FRK_DTYPE_F32, FRK_DTYPE_BF16, FRK_DTYPE_FP8E4M3, FRK_DTYPE_F16X2 = 0, 1, 2, 3
 FILTER_MATCH_ALL = 1
 SELECT_MAX_K = 4096
 

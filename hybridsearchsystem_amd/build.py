@@ -31,7 +31,7 @@ DEVICE = ["--offload-arch=" + ARCH, "-Xclang", "-target-feature", "-Xclang", "-p
 # The scorers keep their MFMA accumulators in ordinary VGPRs: the epilogue reads every accumulator
 # once per doc block, and from AGPRs each read would be an extra copy instruction.
 PER_FILE = {name: ["-mllvm", "-amdgpu-mfma-vgpr-form"]
-            for name in ("frk_score_f32.hip", "frk_score_bf16.hip", "frk_score_fp8.hip")}
+            for name in ("frk_score_f32.hip", "frk_score_bf16.hip", "frk_score_fp8.hip", "frk_score_f16x2.hip")}
 
 
 def _sources():

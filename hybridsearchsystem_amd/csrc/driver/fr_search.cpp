@@ -13,7 +13,7 @@
 // disturbance, CPU affinity. Shards given with --shard_index_dirs all live on --device_id and are
 // searched concurrently (one thread, workspace and stream each).
 //
-// Extra flags of this build: --dtype f32|bf16|fp8 (+ --rescore_factor m), --probe, --graph,
+// Extra flags of this build: --dtype f32|f16x2|bf16|fp8 (+ --rescore_factor m), --probe, --graph,
 // --split_extract (BatchSearchDevice + BatchSearchExtract instead of BatchSearch), --unfused (the
 // reference's unfused order: BatchScore writes the score matrix, then OneFilter + OneAggregator per
 // query), --result_file (per-query ids and scores), --pool_stats.
@@ -632,7 +632,10 @@ int main(int argc, char** argv) {
     r.vecField = F.Str("vec_field", "content");
     r.postField = F.Str("posting_field", "content");
     const std::string dt = F.Str("dtype", "f32");
-    r.opts.dtype = dt == "bf16" ? FRK_DTYPE_BF16 : dt == "fp8" ? FRK_DTYPE_FP8E4M3 : FRK_DTYPE_F32;
+    r.opts.dtype = dt == "bf16"    ? FRK_DTYPE_BF16
+                   : dt == "fp8"   ? FRK_DTYPE_FP8E4M3
+                   : dt == "f16x2" ? FRK_DTYPE_F16X2
+                                   : FRK_DTYPE_F32;
     r.opts.rescoreFactor = static_cast<uint32_t>(F.Int("rescore_factor", 4));
     r.opts.probe = F.Bool("probe", true);
     r.opts.graph = F.Bool("graph", false);

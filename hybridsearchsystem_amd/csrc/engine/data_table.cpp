@@ -131,7 +131,8 @@ VectorFieldData::VectorFieldData(std::string name, uint32_t dim, uint64_t docNum
 VectorFieldData::~VectorFieldData() {
     for (void* p : m_tiles)
         if (p) (void)hipFree(p);
-    if (m_scales) (void)hipFree(m_scales);
+    for (float* p : m_scales)
+        if (p) (void)hipFree(p);
     if (m_rows) (void)hipFree(m_rows);
     if (m_staging) (void)hipFree(m_staging);
 }
@@ -141,7 +142,7 @@ uint32_t VectorFieldData::PaddedDim(uint32_t dtype) const { return DimPad(m_dim,
 bool VectorFieldData::Allocate(int32_t deviceId, uint32_t dtype_mask) {
     (void)deviceId;
     dtype_mask |= 1u << FRK_DTYPE_F32;
-    for (uint32_t dt = 0; dt < 3; dt++) {
+    for (uint32_t dt = 0; dt < 4; dt++) {
         if (!(dtype_mask & (1u << dt))) continue;
         const uint32_t dp = DimPad(m_dim, dt);
         m_bytes[dt] = frk_tiles_bytes(m_docNum, dp, dt);
@@ -150,12 +151,13 @@ bool VectorFieldData::Allocate(int32_t deviceId, uint32_t dtype_mask) {
             m_tiles[dt] = nullptr;
             return false;
         }
-        if (dt == FRK_DTYPE_FP8E4M3 && hipMalloc(reinterpret_cast<void**>(&m_scales), m_docNum * 4 + 4) != hipSuccess) {
-            m_scales = nullptr;
+        const bool scaled = dt == FRK_DTYPE_FP8E4M3 || dt == FRK_DTYPE_F16X2;
+        if (scaled && hipMalloc(reinterpret_cast<void**>(&m_scales[dt]), m_docNum * 4 + 4) != hipSuccess) {
+            m_scales[dt] = nullptr;
             return false;
         }
         m_store[dt].tiles = m_tiles[dt];
-        m_store[dt].scales = dt == FRK_DTYPE_FP8E4M3 ? m_scales : nullptr;
+        m_store[dt].scales = m_scales[dt];
         m_store[dt].dtype = dt;
         m_store[dt].dim = dp;
         m_store[dt].n_docs = m_docNum;
@@ -179,9 +181,9 @@ bool VectorFieldData::SetRows(const float* rows, uint64_t first, uint64_t n, boo
     }
     hipStream_t s = static_cast<hipStream_t>(stream);
     auto tile = [&](const float* d_rows, uint64_t at, uint64_t cnt) {
-        for (uint32_t dt = 0; dt < 3; dt++) {
+        for (uint32_t dt = 0; dt < 4; dt++) {
             if (!m_tiles[dt]) continue;
-            if (frk_tile_rows_device(d_rows, cnt, m_dim, at, dt, m_store[dt].dim, m_tiles[dt], m_scales, s) != 0) {
+            if (frk_tile_rows_device(d_rows, cnt, m_dim, at, dt, m_store[dt].dim, m_tiles[dt], m_scales[dt], s) != 0) {
                 FRK_LOG_ERROR("tiling failed: " << frk_last_error());
                 return false;
             }
@@ -216,12 +218,14 @@ bool VectorFieldData::SetRows(const float* rows, uint64_t first, uint64_t n, boo
 }
 
 const frk_doc_store* VectorFieldData::Store(uint32_t dtype) const {
-    if (dtype > 2 || !m_tiles[dtype]) return nullptr;
+    if (dtype > FRK_DTYPE_F16X2 || !m_tiles[dtype]) return nullptr;
     return &m_store[dtype];
 }
 
 uint64_t VectorFieldData::DeviceBytes() const {
-    return m_bytes[0] + m_bytes[1] + m_bytes[2] + (m_scales ? m_docNum * 4 : 0) + m_rowsBytes;
+    uint64_t b = m_rowsBytes;
+    for (uint32_t dt = 0; dt < 4; dt++) b += m_bytes[dt] + (m_scales[dt] ? m_docNum * 4 : 0);
+    return b;
 }
 
 // ------------------------------------------------------------------------------------------ table

@@ -101,14 +101,14 @@ class VectorFieldData {
     std::string m_name;
     uint32_t m_dim;
     uint64_t m_docNum;
-    frk_doc_store m_store[3] = {};
-    void* m_tiles[3] = {nullptr, nullptr, nullptr};
-    float* m_scales = nullptr;
+    frk_doc_store m_store[4] = {};  // by FRK_DTYPE_*
+    void* m_tiles[4] = {nullptr, nullptr, nullptr, nullptr};
+    float* m_scales[4] = {nullptr, nullptr, nullptr, nullptr};  // per-doc scales (FP8, F16X2)
     float* m_rows = nullptr;
     uint64_t m_rowsBytes = 0;
     float* m_staging = nullptr;  // device rows staging for host uploads
     uint64_t m_stagingRows = 0;
-    uint64_t m_bytes[3] = {0, 0, 0};
+    uint64_t m_bytes[4] = {0, 0, 0, 0};
 };
 
 // ldocid -> gdocid (index/docid/doc_id_mapping.cpp:37-166).

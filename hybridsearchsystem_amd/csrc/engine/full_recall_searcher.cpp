@@ -510,7 +510,9 @@ bool FullRecallSearcher::RunDevice(const VectorFieldData& vf, std::vector<AggrDe
         return false;
     }
     const uint64_t maskStride = static_cast<uint64_t>(segs) * (dnps / 64);
-    const bool lowp = dtype != FRK_DTYPE_F32;
+    // BF16 and FP8 stores select k * m candidates by their own scores and rescore them in FP32; the FP32
+    // store and its F16X2 split score to FP32 accuracy and select the k directly
+    const bool lowp = dtype == FRK_DTYPE_BF16 || dtype == FRK_DTYPE_FP8E4M3;
     const uint32_t m = lowp ? std::max(1u, m_options.rescoreFactor) : 1;
     uint32_t kmax = 1;
     for (uint32_t k : m_topKs) kmax = std::max(kmax, k);

@@ -240,7 +240,7 @@ void SearchServer::RunBatch(Batch& b) {
     }
     uint64_t docBytes = 0;
     if (const frk_doc_store* store = m_vf ? m_vf->Store(m_cfg.options.dtype) : nullptr) {
-        const uint64_t elem = store->dtype == FRK_DTYPE_F32 ? 4 : store->dtype == FRK_DTYPE_BF16 ? 2 : 1;
+        const uint64_t elem = store->dtype == FRK_DTYPE_BF16 ? 2 : store->dtype == FRK_DTYPE_FP8E4M3 ? 1 : 4;
         docBytes = (store->n_docs + 31) / 32 * 32 * store->dim * elem * (st.reruns ? 2 : 1);
     }
     std::lock_guard<std::mutex> g(m_mu);

@@ -112,4 +112,59 @@ __host__ __device__ inline float Fp8Scale(float mx) {
     return ldexpf(1.0f, e);
 }
 
+// IEEE binary16, round to nearest even, subnormals kept, overflow to infinity, NaN stays NaN. One
+// software definition for the host and the device, so stores tiled on either side are bit-identical.
+__host__ __device__ inline uint16_t F32ToF16(float f) {
+    const uint32_t u = __builtin_bit_cast(uint32_t, f);
+    const uint16_t sign = static_cast<uint16_t>((u >> 16) & 0x8000u);
+    const uint32_t a = u & 0x7FFFFFFFu;
+    if (a >= 0x7F800000u) return static_cast<uint16_t>(sign | 0x7C00u | (a > 0x7F800000u ? 0x200u : 0u));
+    if (a >= 0x477FF000u) return static_cast<uint16_t>(sign | 0x7C00u);  // rounds to at least 65520: infinity
+    if (a < 0x38800000u) {
+        // below 2^-14: a subnormal half, k * 2^-24 with k = round(|f| * 2^24), ties to even
+        if (a < 0x33000000u) return sign;  // at most 2^-25: ties to even 0
+        const uint32_t m = (a & 0x7FFFFFu) | 0x800000u;
+        const uint32_t shift = 126u - (a >> 23);  // 14..24 for exponents 2^-25 .. 2^-15
+        const uint32_t k = m >> shift, rem = m & ((1u << shift) - 1u), half = 1u << (shift - 1u);
+        const uint32_t r = k + (rem > half || (rem == half && (k & 1u)));
+        return static_cast<uint16_t>(sign | r);
+    }
+    // normal: rebias the exponent (127 -> 15), keep 10 mantissa bits, round to nearest even
+    const uint32_t b = a - 0x38000000u;
+    const uint32_t r = (b + 0xFFFu + ((b >> 13) & 1u)) >> 13;
+    return static_cast<uint16_t>(sign | r);
+}
+
+// IEEE binary16 bits -> FP32 (exact).
+__host__ __device__ inline float F16ToF32(uint16_t h) {
+    const uint32_t sign = static_cast<uint32_t>(h & 0x8000u) << 16;
+    const uint32_t e = (h >> 10) & 0x1Fu, m = h & 0x3FFu;
+    if (e == 0) {
+        const float v = ldexpf(static_cast<float>(m), -24);
+        return sign ? -v : v;
+    }
+    if (e == 31) return __builtin_bit_cast(float, sign | 0x7F800000u | (m << 13));
+    return __builtin_bit_cast(float, sign | ((e + 112u) << 23) | (m << 13));
+}
+
+// F16X2 scale of a vector whose largest magnitude is mx: the power of two s = 2^e with mx * s in
+// [2^14, 2^15), so hi and lo of every element keep FP16's 11-bit precision down to 2^-17 of mx (below
+// that the halves are subnormal, with an absolute error under 2^-25 / s); e clamped to [-126, 126];
+// 1 for a zero vector.
+__host__ __device__ inline float SplitScale(float mx) {
+    if (!(mx > 0.0f)) return 1.0f;
+    if (!(mx <= 3.402823466e38f)) return ldexpf(1.0f, -126);
+    int E;
+    (void)frexpf(mx, &E);  // mx = m * 2^E, m in [0.5, 1): mx * 2^(15 - E) in [2^14, 2^15)
+    int e = 15 - E;
+    e = e < -126 ? -126 : (e > 126 ? 126 : e);
+    return ldexpf(1.0f, e);
+}
+
+// The hi and lo halves of v (already scaled): v = F16ToF32(hi) + F16ToF32(lo) within 2^-11 |v - hi|.
+__host__ __device__ inline void SplitF16(float v, uint16_t& hi, uint16_t& lo) {
+    hi = F32ToF16(v);
+    lo = F32ToF16(v - F16ToF32(hi));
+}
+
 }  // namespace frk

@@ -1,5 +1,6 @@
 // frk_layout.cpp -- host-side conversion of row-major FP32 vectors into the MFMA-native doc tiling
-// (frk_capi.h, "Doc store"), with BF16 or FP8 E4M3 (+ per-doc FP32 scale) quantisation. Used by the
+// (frk_capi.h, "Doc store"), with BF16 or FP8 E4M3 (+ per-doc FP32 scale) quantisation, or the
+// F16X2 hi/lo split (+ per-doc scale). Used by the
 // index loader (DataTable) and exported for callers that stage their own stores.
 #include <string.h>
 
@@ -26,12 +27,28 @@ void TileRange(const float* src, uint64_t n_docs, uint32_t dim, uint32_t dtype, 
             std::fill(row.begin(), row.end(), 0.0f);
             if (n < n_docs) memcpy(row.data(), src + n * dim, sizeof(float) * dim);
             float inv = 1.0f;
-            if (dtype == FRK_DTYPE_FP8E4M3) {
+            if (dtype == FRK_DTYPE_FP8E4M3 || dtype == FRK_DTYPE_F16X2) {
                 float mx = 0.0f;
                 for (uint32_t d = 0; d < dim; d++) mx = std::max(mx, std::fabs(row[d]));
-                const float sc = frk::Fp8Scale(mx);
+                const float sc = dtype == FRK_DTYPE_FP8E4M3 ? frk::Fp8Scale(mx) : frk::SplitScale(mx);
                 if (n < n_docs) scales[n] = sc;
                 inv = 1.0f / sc;
+            }
+            if (dtype == FRK_DTYPE_F16X2) {
+                // chunk pair p: the hi halves (chunk 2p) and lo halves (chunk 2p + 1) of dims 16 p + 8 h + e
+                const float s = 1.0f / inv;
+                for (uint32_t c = 0; c < chunks; c++) {
+                    for (uint32_t h = 0; h < 2; h++) {
+                        uint16_t* o = reinterpret_cast<uint16_t*>(dst + ((b * chunks + c) * 64 + (j + 32 * h)) * 16);
+                        const float* v = row.data() + (c / 2) * 16 + h * 8;
+                        for (uint32_t e = 0; e < 8; e++) {
+                            uint16_t hi, lo;
+                            SplitF16(v[e] * s, hi, lo);
+                            o[e] = (c % 2) ? lo : hi;
+                        }
+                    }
+                }
+                continue;
             }
             for (uint32_t c = 0; c < chunks; c++) {
                 for (uint32_t h = 0; h < 2; h++) {
@@ -67,8 +84,8 @@ extern "C" int frk_tile_docs(const float* src, uint64_t n_docs, uint32_t dim, ui
         frk::SetError("frk_tile_docs: bad arguments (dim %u, dim_pad %u, dtype %u)", dim, dim_pad, dtype);
         return FRK_ERR_INVALID;
     }
-    if (dtype == FRK_DTYPE_FP8E4M3 && !dst_scales) {
-        frk::SetError("frk_tile_docs: FP8 needs a scale array");
+    if ((dtype == FRK_DTYPE_FP8E4M3 || dtype == FRK_DTYPE_F16X2) && !dst_scales) {
+        frk::SetError("frk_tile_docs: FP8 and F16X2 need a scale array");
         return FRK_ERR_INVALID;
     }
     const uint64_t n_blocks = (n_docs + 31) / 32;

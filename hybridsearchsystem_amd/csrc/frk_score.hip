@@ -1,5 +1,5 @@
 // frk_score.hip -- C ABI of the fused scorer: argument checks, launch shape, dtype dispatch. The
-// kernel is in frk_score_impl.h, instantiated per dtype in frk_score_{f32,bf16,fp8}.hip.
+// kernel is in frk_score_impl.h, instantiated per dtype in frk_score_{f32,bf16,fp8,f16x2}.hip.
 #include <atomic>
 #include <string>
 
@@ -51,6 +51,10 @@ int Prepare(const frk_doc_store* store, const float* d_queries, uint32_t n_queri
     const uint32_t ch = store->dim / cd;
     if (store->dtype == FRK_DTYPE_FP8E4M3 && (!store->scales || ch % 2 != 0)) {
         SetError("%s: FP8 needs per-doc scales and a dim that is a multiple of 64", who);
+        return FRK_ERR_INVALID;
+    }
+    if (store->dtype == FRK_DTYPE_F16X2 && (!store->scales || ch % 2 != 0)) {
+        SetError("%s: F16X2 needs per-doc scales and a dim that is a multiple of 16", who);
         return FRK_ERR_INVALID;
     }
     const uint64_t n_blocks = (store->n_docs + 31) / 32;
@@ -106,6 +110,7 @@ int Run(const frk_doc_store* store, const ScoreLaunch& L, int mode) {
         case FRK_DTYPE_F32: return ScoreF32(L, mode);
         case FRK_DTYPE_BF16: return ScoreBf16(L, mode);
         case FRK_DTYPE_FP8E4M3: return ScoreFp8(L, mode);
+        case FRK_DTYPE_F16X2: return ScoreF16x2(L, mode);
     }
     SetError("unknown dtype %u", store->dtype);
     return FRK_ERR_INVALID;
@@ -212,6 +217,7 @@ extern "C" uint32_t frk_chunk_dims(uint32_t dtype) {
         case FRK_DTYPE_F32: return 8;
         case FRK_DTYPE_BF16: return 16;
         case FRK_DTYPE_FP8E4M3: return 32;
+        case FRK_DTYPE_F16X2: return 8;  // a chunk pair (hi, lo) holds 16 dims
     }
     return 0;
 }
@@ -219,7 +225,8 @@ extern "C" uint32_t frk_chunk_dims(uint32_t dtype) {
 extern "C" uint32_t frk_store_dim(uint32_t dim, uint32_t dtype) {
     const uint32_t cd = frk_chunk_dims(dtype);
     if (cd == 0 || dim == 0) return 0;
-    const uint32_t step = dtype == FRK_DTYPE_FP8E4M3 ? 2 : 1;  // one FP8 MFMA step reads two chunks
+    // one FP8 MFMA step reads two chunks, and an F16X2 slice is a chunk pair
+    const uint32_t step = dtype == FRK_DTYPE_FP8E4M3 || dtype == FRK_DTYPE_F16X2 ? 2 : 1;
     uint32_t ch = (dim + cd - 1) / cd;
     ch = (ch + step - 1) / step * step;
     for (uint32_t s : frk::kScoreShapes)

@@ -59,7 +59,7 @@ struct WgShape {
     // ms, BF16 scorer alone 0.94 -> 0.91 ms).
     static constexpr int kThreads =
         (DT != FRK_DTYPE_BF16 || RB * CH <= 64) &&
-                RB * CH > (DT == FRK_DTYPE_F32 ? FRK_SCORE_WIDE_TILE : FRK_SCORE_WIDE_TILE_FP8) &&
+                RB * CH > (DT == FRK_DTYPE_F32 || DT == FRK_DTYPE_F16X2 ? FRK_SCORE_WIDE_TILE : FRK_SCORE_WIDE_TILE_FP8) &&
                 RB * CH <= (DT == FRK_DTYPE_FP8E4M3 ? 96 : 128)
             ? 512
             : 256;
@@ -84,6 +84,15 @@ template <>
 struct Traits<FRK_DTYPE_FP8E4M3> {
     static constexpr int kElemsPerLane = 16;
 };
+template <>
+struct Traits<FRK_DTYPE_F16X2> {
+    static constexpr int kElemsPerLane = 8;  // halves per lane: the hi (even chunk) or lo (odd chunk) parts
+};
+
+// Stores whose docs carry a per-doc power-of-two scale that the accumulators still hold (F16X2: the
+// MFMAs multiply scaled halves, so a score is acc / (query scale * doc scale)).
+template <int DT>
+constexpr bool kUnscale = DT == FRK_DTYPE_F16X2;
 
 // One chunk (or, for fp8, one chunk pair) of MFMA work for one 32-query row block.
 template <int DT>
@@ -104,6 +113,12 @@ __device__ __forceinline__ void MfmaChunk<FRK_DTYPE_BF16>(const u32x4& a, const 
     acc = __builtin_amdgcn_mfma_f32_32x32x16_bf16(__builtin_bit_cast(bf16x8, a), __builtin_bit_cast(bf16x8, b), acc,
                                                   0, 0, 0);
 }
+
+typedef _Float16 f16x8 __attribute__((ext_vector_type(8)));
+
+// One 16-dim F16X2 slice for one row block: (query hi, query lo) x (doc hi, doc lo) as three FP16 MFMAs,
+// hi hi + hi lo + lo hi (lo lo is below 2^-21 of the product and dropped).
+__device__ __forceinline__ f16x8 AsF16(const u32x4& v) { return __builtin_bit_cast(f16x8, v); }
 
 // sa, sb: E8M0 scale exponents (2^(e - 127)) of this lane's query row and doc column
 __device__ __forceinline__ void MfmaChunkPairFp8(const u32x4& a0, const u32x4& a1, const u32x4& b0, const u32x4& b1,
@@ -142,8 +157,8 @@ struct ScoreArgs {
     uint32_t n_groups;  // query groups of 32 RB rows (set by the launch)
 };
 
-// Writes the query tile into LDS in the doc tiling, converted to the store's element type; for fp8 the
-// per-query power-of-two scale (Fp8Scale of max|q|) goes to q_scale_lds. Every workgroup of a launch
+// Writes the query tile into LDS in the doc tiling, converted to the store's element type; for fp8 and
+// F16X2 the per-query power-of-two scale (Fp8Scale or SplitScale of max|q|) goes to q_scale_lds. Every workgroup of a launch
 // does this, so it is cheap by construction: a query's max|q| is one wave's work (a 16-byte load per lane
 // and a shuffle reduction), and each tile entry reads its E query values with 16-byte loads (query rows
 // are 16-byte aligned and the padded dim is a multiple of 8).
@@ -151,7 +166,7 @@ template <int DT, int CH, int RB>
 __device__ __forceinline__ void StageQueries(const ScoreArgs& A, u32x4* a_lds, float* q_scale_lds, uint32_t q0) {
     constexpr int E = Traits<DT>::kElemsPerLane;
     constexpr int kThreads = WgShape<DT, CH, RB>::kThreads;
-    if constexpr (DT == FRK_DTYPE_FP8E4M3) {
+    if constexpr (DT == FRK_DTYPE_FP8E4M3 || DT == FRK_DTYPE_F16X2) {
         const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
         for (int r = wave; r < 32 * RB; r += kThreads / 64) {
             const uint32_t q = q0 + r;
@@ -164,7 +179,7 @@ __device__ __forceinline__ void StageQueries(const ScoreArgs& A, u32x4* a_lds, f
                 }
             }
             for (int off = 32; off > 0; off >>= 1) mx = fmaxf(mx, __shfl_xor(mx, off, 64));
-            if (lane == 0) q_scale_lds[r] = Fp8Scale(mx);
+            if (lane == 0) q_scale_lds[r] = DT == FRK_DTYPE_FP8E4M3 ? Fp8Scale(mx) : SplitScale(mx);
         }
         __syncthreads();
     }
@@ -174,7 +189,8 @@ __device__ __forceinline__ void StageQueries(const ScoreArgs& A, u32x4* a_lds, f
         const int rb = idx / (CH * 64);
         const int i = l & 31, h = l >> 5;
         const uint32_t q = q0 + 32 * rb + i;
-        const uint32_t d0 = c * 2 * E + h * E;
+        // F16X2: chunk pair c / 2 is the 16-dim slice, the even chunk its hi halves and the odd its lo
+        const uint32_t d0 = DT == FRK_DTYPE_F16X2 ? (c / 2) * 16 + h * 8 : c * 2 * E + h * E;
         float v[E];
         if (q < A.n_queries) {
             const floatx4* src = reinterpret_cast<const floatx4*>(A.queries + static_cast<uint64_t>(q) * A.q_dim + d0);
@@ -195,6 +211,18 @@ __device__ __forceinline__ void StageQueries(const ScoreArgs& A, u32x4* a_lds, f
         } else if constexpr (DT == FRK_DTYPE_BF16) {
             for (int e = 0; e < 4; e++)
                 pw[e] = static_cast<uint32_t>(F32ToBf16(v[2 * e])) | (static_cast<uint32_t>(F32ToBf16(v[2 * e + 1])) << 16);
+        } else if constexpr (DT == FRK_DTYPE_F16X2) {
+            // v * scale (a power of two, exact) split into hi = fp16(v s) and lo = fp16(v s - hi), with the
+            // hardware's round-to-nearest-even conversion
+            const float sc = q_scale_lds[32 * rb + i];
+            f16x8 part;
+#pragma unroll
+            for (int e = 0; e < 8; e++) {
+                const float x = v[e] * sc;
+                const _Float16 hi = static_cast<_Float16>(x);
+                part[e] = (c & 1) ? static_cast<_Float16>(x - static_cast<float>(hi)) : hi;
+            }
+            packed = __builtin_bit_cast(u32x4, part);
         } else {
             // v / scale lies within [-448, 448] (the scale is a power of two chosen for that), where the
             // hardware conversion rounds exactly as F32ToE4M3 does (nearest even, subnormals included;
@@ -267,6 +295,21 @@ __device__ __forceinline__ void ComputeGroup(FragA&& a, const u32x4 (&buf)[Group
 #pragma unroll
             for (int rb = 0; rb < RB; rb++)
                 MfmaChunkPairFp8(a(rb, c), a(rb, c + 1), buf[c], buf[c + 1], acc[rb], qexp[rb], dexp);
+    } else if constexpr (DT == FRK_DTYPE_F16X2) {
+        // the three products of a slice are issued row block by row block, so consecutive MFMAs of a
+        // row block's chain are RB instructions apart
+#pragma unroll
+        for (int c = 0; c < G; c += 2) {
+#pragma unroll
+            for (int rb = 0; rb < RB; rb++)
+                acc[rb] = __builtin_amdgcn_mfma_f32_32x32x16_f16(AsF16(a(rb, c)), AsF16(buf[c]), acc[rb], 0, 0, 0);
+#pragma unroll
+            for (int rb = 0; rb < RB; rb++)
+                acc[rb] = __builtin_amdgcn_mfma_f32_32x32x16_f16(AsF16(a(rb, c)), AsF16(buf[c + 1]), acc[rb], 0, 0, 0);
+#pragma unroll
+            for (int rb = 0; rb < RB; rb++)
+                acc[rb] = __builtin_amdgcn_mfma_f32_32x32x16_f16(AsF16(a(rb, c + 1)), AsF16(buf[c]), acc[rb], 0, 0, 0);
+        }
     } else {
 #pragma unroll
         for (int c = 0; c < G; c++)
@@ -297,16 +340,23 @@ __device__ __forceinline__ void WaveSync() {
     __builtin_amdgcn_wave_barrier();
 }
 
-__device__ __forceinline__ void FlushQueue(const ScoreArgs& A, uint32_t q0, WaveQueue& Qu) {
+// 1 / s for a power of two s = 2^e, e in [-126, 126]: the exponent field mirrored around the bias (exact).
+__device__ __forceinline__ float InvPow2(float s) { return __uint_as_float(0x7F000000u - __float_as_uint(s)); }
+
+// q_scale: the per-row query scales whose factor an entry's score still holds (F16X2), else null.
+__device__ __forceinline__ void FlushQueue(const ScoreArgs& A, uint32_t q0, WaveQueue& Qu, const float* q_scale) {
     const int lane = threadIdx.x & 63;
     WaveSync();
     for (uint32_t i0 = 0; i0 < Qu.n; i0 += 64) {
         const uint32_t i = i0 + lane;
         if (i < Qu.n) {
             const uint2 e = Qu.ent[i];
-            const uint32_t q = q0 + Qu.row[i];
+            const uint32_t r = Qu.row[i];
+            const uint32_t q = q0 + r;
+            float sc = __uint_as_float(e.x);
+            if (q_scale) sc *= InvPow2(q_scale[r]);
             const uint32_t at = atomicAdd(&A.counts[static_cast<uint64_t>(q) * A.counts_stride], 1u);
-            if (at < A.cap) A.cand[static_cast<uint64_t>(q) * A.cap + at] = MakeKey(__uint_as_float(e.x), e.y);
+            if (at < A.cap) A.cand[static_cast<uint64_t>(q) * A.cap + at] = MakeKey(sc, e.y);
         }
     }
     WaveSync();
@@ -315,12 +365,12 @@ __device__ __forceinline__ void FlushQueue(const ScoreArgs& A, uint32_t q0, Wave
 
 // The block's filter words for this wave's row blocks: for row block rb, lane l loads the word of doc
 // (l % 32) whose bit x says whether query q0 + 32 rb + x passes it (one coalesced 128-byte load per row
-// block), together with the block's first group of doc tiles; for FP8 also the doc's scale.
+// block), together with the block's first group of doc tiles; for FP8 and F16X2 also the doc's scale.
 template <int DT, int RB>
 __device__ __forceinline__ void LoadMasks(const ScoreArgs& A, uint32_t q0, uint64_t blk, uint32_t (&m)[RB],
                                           float& dscale) {
     const int lane = threadIdx.x & 63;
-    if constexpr (DT == FRK_DTYPE_FP8E4M3) {
+    if constexpr (DT == FRK_DTYPE_FP8E4M3 || DT == FRK_DTYPE_F16X2) {
         // the doc's scale (a doc past the end reads the last doc's; its scores are NaN anyway)
         const uint64_t doc = blk * 32 + (lane & 31);
         dscale = A.doc_scale[doc < A.n_docs ? doc : A.n_docs - 1];
@@ -363,12 +413,18 @@ __device__ __forceinline__ uint32_t AccRow(int rb, int r, uint32_t half) {
 }
 
 // One row block of the epilogue; rb is a template parameter so the accumulator is always indexed
-// statically (a runtime row-block index would send it to scratch).
-template <int RB, int rb>
-__device__ __forceinline__ void EpilogueRowBlock(const ScoreArgs& A, const float* floor_lds, uint32_t q0, uint64_t doc,
-                                        const floatx16& accr, WaveQueue& Qu) {
+// statically (a runtime row-block index would send it to scratch). With kUnscale<DT> the accumulators
+// hold score * query scale * doc scale: dinv = 1 / doc scale is applied here (exactly, a power of two),
+// the floors in floor_lds are already multiplied by the query scale, and the flush divides the query
+// scale out of each survivor's score.
+template <int DT, int RB, int rb>
+__device__ __forceinline__ void EpilogueRowBlock(const ScoreArgs& A, const float* floor_lds, const float* q_scale_lds,
+                                                 uint32_t q0, uint64_t doc, float dinv, const floatx16& accr,
+                                                 WaveQueue& Qu) {
     const int lane = threadIdx.x & 63;
     const uint32_t half = lane >> 5;
+    auto diff = [&](float a, float f) { return kUnscale<DT> ? fmaf(a, dinv, -f) : a - f; };
+    auto score = [&](float a) { return kUnscale<DT> ? a * dinv : a; };
     // Does any score of the row block reach its row's floor? The largest (score - floor) over the lane's
     // 16 rows, reduced with three-operand maxima, then one ballot: filtered docs hold NaN, which the
     // maximum passes over (a row with nothing but NaN stays NaN and fails the test), and a score at or
@@ -378,8 +434,8 @@ __device__ __forceinline__ void EpilogueRowBlock(const ScoreArgs& A, const float
 #pragma unroll
     for (int g = 0; g < 4; g++) {
         const floatx4 f4 = *reinterpret_cast<const floatx4*>(floor_lds + 32 * rb + 8 * g + 4 * half);
-        mg[g] = fmaxf(fmaxf(accr[4 * g] - f4[0], accr[4 * g + 1] - f4[1]),
-                      fmaxf(accr[4 * g + 2] - f4[2], accr[4 * g + 3] - f4[3]));
+        mg[g] = fmaxf(fmaxf(diff(accr[4 * g], f4[0]), diff(accr[4 * g + 1], f4[1])),
+                      fmaxf(diff(accr[4 * g + 2], f4[2]), diff(accr[4 * g + 3], f4[3])));
     }
     if (__ballot(fmaxf(fmaxf(mg[0], mg[1]), fmaxf(mg[2], mg[3])) >= 0.0f) == 0) return;
     // Survivors, four rows at a time: one room check per group for at most 4 per lane that holds any
@@ -390,39 +446,46 @@ __device__ __forceinline__ void EpilogueRowBlock(const ScoreArgs& A, const float
     for (int g = 0; g < 4; g++) {
         const uint64_t any = __ballot(mg[g] >= 0.0f);
         if (any == 0) continue;
-        if (Qu.n + 4u * static_cast<uint32_t>(__popcll(any)) > static_cast<uint32_t>(kQueueKeys)) FlushQueue(A, q0, Qu);
+        if (Qu.n + 4u * static_cast<uint32_t>(__popcll(any)) > static_cast<uint32_t>(kQueueKeys))
+            FlushQueue(A, q0, Qu, kUnscale<DT> ? q_scale_lds : nullptr);
         const floatx4 f4 = *reinterpret_cast<const floatx4*>(floor_lds + 32 * rb + 8 * g + 4 * half);
 #pragma unroll
         for (int x = 0; x < 4; x++) {
             const int r = 4 * g + x;
-            const bool hit = accr[r] >= f4[x];
+            const float sv = score(accr[r]);
+            const bool hit = sv >= f4[x];
             const uint64_t bal = __ballot(hit);
             if (bal == 0) continue;
             if (hit) {
                 const uint32_t below = __builtin_amdgcn_mbcnt_hi(static_cast<uint32_t>(bal >> 32),
                                                                  __builtin_amdgcn_mbcnt_lo(static_cast<uint32_t>(bal), 0u));
                 const uint32_t slot = Qu.n + below;
-                Qu.ent[slot] = make_uint2(__float_as_uint(accr[r]), d32);
-                Qu.row[slot] = static_cast<uint8_t>(AccRow(rb, r, half));
+                Qu.ent[slot] = make_uint2(__float_as_uint(sv), d32);
+                // the lane half's row offset, opaque here so that the compiler does not hoist all 16 RB
+                // row indices out of the loop into registers of their own
+                uint32_t h4 = 4u * half;
+                asm volatile("" : "+v"(h4));
+                Qu.row[slot] = static_cast<uint8_t>(AccRow(rb, r, 0) + h4);
             }
             Qu.n += static_cast<uint32_t>(__popcll(bal));
         }
     }
 }
 
-template <int RB, int... I>
-__device__ __forceinline__ void EpilogueAll(const ScoreArgs& A, const float* floor_lds, uint32_t q0, uint64_t doc,
-                                   const floatx16 (&acc)[RB], WaveQueue& Qu, std::integer_sequence<int, I...>) {
-    (EpilogueRowBlock<RB, I>(A, floor_lds, q0, doc, acc[I], Qu), ...);
+template <int DT, int RB, int... I>
+__device__ __forceinline__ void EpilogueAll(const ScoreArgs& A, const float* floor_lds, const float* q_scale_lds,
+                                            uint32_t q0, uint64_t doc, float dinv, const floatx16 (&acc)[RB],
+                                            WaveQueue& Qu, std::integer_sequence<int, I...>) {
+    (EpilogueRowBlock<DT, RB, I>(A, floor_lds, q_scale_lds, q0, doc, dinv, acc[I], Qu), ...);
 }
 
 // Epilogue of one finished 32-doc block: every accumulator row against its floor (filtered docs and
 // docs past the end hold NaN); survivors go to the wave queue.
-template <int RB>
-__device__ __forceinline__ void Epilogue(const ScoreArgs& A, const float* floor_lds, uint32_t q0, uint64_t blk,
-                                const floatx16 (&acc)[RB], WaveQueue& Qu) {
+template <int DT, int RB>
+__device__ __forceinline__ void Epilogue(const ScoreArgs& A, const float* floor_lds, const float* q_scale_lds, uint32_t q0,
+                                         uint64_t blk, float dinv, const floatx16 (&acc)[RB], WaveQueue& Qu) {
     const uint64_t doc = blk * 32 + (threadIdx.x & 31);
-    EpilogueAll<RB>(A, floor_lds, q0, doc, acc, Qu, std::make_integer_sequence<int, RB>{});
+    EpilogueAll<DT, RB>(A, floor_lds, q_scale_lds, q0, doc, dinv, acc, Qu, std::make_integer_sequence<int, RB>{});
 }
 
 // One stage of the block-max butterfly over lane bit B with N rows left per lane (M = array size):
@@ -458,10 +521,11 @@ __device__ __forceinline__ void BlockMaxStages(float (&v)[M], uint32_t j, uint32
 // 16 RB rows a lane holds shrink to 16 RB / 32 after 16 RB - 1 exchanges in total (instead of five
 // exchanges per row). After it, lane j of half h holds the rows of flat accumulator indices 2j, 2j+1
 // (RB = 4), or the row of index j (RB = 2); with RB = 1 the last stage is a plain exchange of the
-// single remaining row, index j / 2, which lanes j and j ^ 1 then share.
-template <int RB>
-__device__ __forceinline__ void EpilogueBlockMax(const ScoreArgs& A, uint32_t q0, uint64_t step_index,
-                                        const floatx16 (&acc)[RB]) {
+// single remaining row, index j / 2, which lanes j and j ^ 1 then share. With kUnscale<DT> the doc
+// scale is divided out before the maximum and the query scale after it.
+template <int DT, int RB>
+__device__ __forceinline__ void EpilogueBlockMax(const ScoreArgs& A, const float* q_scale_lds, uint32_t q0,
+                                                 uint64_t step_index, float dinv, const floatx16 (&acc)[RB]) {
     constexpr int N = 16 * RB;
     const int lane = threadIdx.x & 63;
     const uint32_t j = lane & 31;
@@ -470,7 +534,7 @@ __device__ __forceinline__ void EpilogueBlockMax(const ScoreArgs& A, uint32_t q0
 #pragma unroll
     for (int rb = 0; rb < RB; rb++)
 #pragma unroll
-        for (int r = 0; r < 16; r++) v[16 * rb + r] = acc[rb][r];
+        for (int r = 0; r < 16; r++) v[16 * rb + r] = kUnscale<DT> ? acc[rb][r] * dinv : acc[rb][r];
     uint32_t base = 0;  // flat index of v[0] after the halving stages
     BlockMaxStages<N, N, 16>(v, j, base);
     constexpr int n = N >= 32 ? N / 32 : 1;  // rows left per lane
@@ -483,15 +547,17 @@ __device__ __forceinline__ void EpilogueBlockMax(const ScoreArgs& A, uint32_t q0
         const uint32_t r = base + e;
         const uint32_t row = 32 * (r >> 4) + 8 * ((r >> 2) & 3) + 4 * half + (r & 3);
         const uint32_t q = q0 + row;
-        if (q < A.n_queries) out[static_cast<uint64_t>(q) * A.cap + step_index] = v[e] == v[e] ? v[e] : -INFINITY;
+        const float m = kUnscale<DT> ? v[e] * InvPow2(q_scale_lds[row]) : v[e];
+        if (q < A.n_queries) out[static_cast<uint64_t>(q) * A.cap + step_index] = m == m ? m : -INFINITY;
     }
 }
 
 // Dense epilogue: every score of the block to the matrix, out[q * A.cap + doc] for docs below n_docs and
 // real queries. For each accumulator register, lanes 0-31 write 32 consecutive docs of one row and
 // lanes 32-63 those of the row four below: two 128-byte runs per store instruction.
-template <int RB>
-__device__ __forceinline__ void EpilogueDense(const ScoreArgs& A, uint32_t q0, uint64_t blk, const floatx16 (&acc)[RB]) {
+template <int DT, int RB>
+__device__ __forceinline__ void EpilogueDense(const ScoreArgs& A, const float* q_scale_lds, uint32_t q0, uint64_t blk,
+                                              float dinv, const floatx16 (&acc)[RB]) {
     const int lane = threadIdx.x & 63;
     const uint64_t doc = blk * 32 + (lane & 31);
     if (doc >= A.n_docs) return;
@@ -501,8 +567,10 @@ __device__ __forceinline__ void EpilogueDense(const ScoreArgs& A, uint32_t q0, u
     for (int rb = 0; rb < RB; rb++)
 #pragma unroll
         for (int r = 0; r < 16; r++) {
-            const uint32_t q = q0 + 32 * rb + 8 * (r / 4) + 4 * half + (r % 4);
-            if (q < A.n_queries) out[static_cast<uint64_t>(q) * A.cap + doc] = acc[rb][r];
+            const uint32_t row = 32 * rb + 8 * (r / 4) + 4 * half + (r % 4);
+            const uint32_t q = q0 + row;
+            const float sc = kUnscale<DT> ? acc[rb][r] * dinv * InvPow2(q_scale_lds[row]) : acc[rb][r];
+            if (q < A.n_queries) out[static_cast<uint64_t>(q) * A.cap + doc] = sc;
         }
 }
 
@@ -538,7 +606,9 @@ __global__ __launch_bounds__((WgShape<DT, CH, RB>::kThreads), 1) void ScoreKerne
     for (int r = threadIdx.x; r < 32 * RB; r += kThreads) {
         const uint32_t q = q0 + r;
         // rows past the last query never pass
-        floor_lds[r] = q < A.n_queries ? (A.floor ? A.floor[q] : -INFINITY) : INFINITY;
+        const float fl = q < A.n_queries ? (A.floor ? A.floor[q] : -INFINITY) : INFINITY;
+        // F16X2 compares scores that still hold the query's scale (a power of two: exact)
+        floor_lds[r] = kUnscale<DT> ? fl * q_scale_lds[r] : fl;
     }
     __syncthreads();
 
@@ -570,6 +640,7 @@ __global__ __launch_bounds__((WgShape<DT, CH, RB>::kThreads), 1) void ScoreKerne
             qexp[rb] = static_cast<int>((__float_as_uint(q_scale_lds[32 * rb + (lane & 31)]) >> 23) & 0xFFu);
     }
     int dexp = 127;
+    float dinv = 1.0f;  // F16X2: 1 / the block's doc scale, per lane
     // staged query fragments (kStageA): two buffers, the step's group and the next step's
     constexpr bool kStageA = WgShape<DT, CH, RB>::kStageA;
     u32x4 afrag[kStageA ? 2 : 1][G][RB];
@@ -615,6 +686,7 @@ __global__ __launch_bounds__((WgShape<DT, CH, RB>::kThreads), 1) void ScoreKerne
             const uint64_t blk = A.block_begin + block_index(step) * A.block_step;
             InitBias<RB>(mring[S], blk * 32 + (lane & 31) < A.n_docs, acc);
             if constexpr (DT == FRK_DTYPE_FP8E4M3) dexp = static_cast<int>((__float_as_uint(sring[S]) >> 23) & 0xFFu);
+            if constexpr (kUnscale<DT>) dinv = InvPow2(sring[S]);
         }
         if constexpr (kStageA) {
             // the next step's fragments are read before this step's MFMAs are issued, so their LDS
@@ -638,11 +710,11 @@ __global__ __launch_bounds__((WgShape<DT, CH, RB>::kThreads), 1) void ScoreKerne
             const uint64_t i = block_index(step);
             const uint64_t blk = A.block_begin + i * A.block_step;
             if constexpr (kMode == kModeBlockMax)
-                EpilogueBlockMax<RB>(A, q0, i, acc);
+                EpilogueBlockMax<DT, RB>(A, q_scale_lds, q0, i, dinv, acc);
             else if constexpr (kMode == kModeDense)
-                EpilogueDense<RB>(A, q0, blk, acc);
+                EpilogueDense<DT, RB>(A, q_scale_lds, q0, blk, dinv, acc);
             else
-                Epilogue<RB>(A, floor_lds, q0, blk, acc, Qu);
+                Epilogue<DT, RB>(A, floor_lds, q_scale_lds, q0, blk, dinv, acc, Qu);
         }
     };
     uint64_t base = 0;
@@ -655,7 +727,7 @@ __global__ __launch_bounds__((WgShape<DT, CH, RB>::kThreads), 1) void ScoreKerne
             if (base + decltype(s_c)::value < total) step_fn(s_c, base);
         }, std::make_integer_sequence<int, U - NG>{});
     }
-    if constexpr (kMode == kModeSelect) FlushQueue(A, q0, Qu);
+    if constexpr (kMode == kModeSelect) FlushQueue(A, q0, Qu, kUnscale<DT> ? q_scale_lds : nullptr);
 }
 
 using KernelFn = void (*)(ScoreArgs);
@@ -765,4 +837,5 @@ struct ScoreLaunch {
 int ScoreF32(const ScoreLaunch& L, int mode);
 int ScoreBf16(const ScoreLaunch& L, int mode);
 int ScoreFp8(const ScoreLaunch& L, int mode);
+int ScoreF16x2(const ScoreLaunch& L, int mode);
 }  // namespace frk

@@ -1,5 +1,5 @@
 // frk_tile.hip -- device-side tiling of row-major FP32 vectors into a doc store (frk_capi.h, "Doc
-// store"), with BF16 / FP8 E4M3 (+ per-doc scale) conversion. The index loader and builder stream
+// store"), with BF16 / FP8 E4M3 (+ per-doc scale) conversion or the F16X2 hi/lo split (+ per-doc scale). The index loader and builder stream
 // rows through it in chunks, so the FP32 originals never need a second resident copy.
 #include "frk_common.h"
 
@@ -7,14 +7,14 @@ namespace frk {
 namespace {
 
 __global__ __launch_bounds__(256) void RowScaleKernel(const float* __restrict__ rows, uint64_t n, uint32_t dim,
-                                                      float* __restrict__ scales) {
+                                                      uint32_t dtype, float* __restrict__ scales) {
     const uint64_t r = static_cast<uint64_t>(blockIdx.x) * 4 + (threadIdx.x >> 6);
     const int lane = threadIdx.x & 63;
     if (r >= n) return;
     float mx = 0.0f;
     for (uint32_t d = lane; d < dim; d += 64) mx = fmaxf(mx, fabsf(rows[r * dim + d]));
     for (int off = 32; off > 0; off >>= 1) mx = fmaxf(mx, __shfl_xor(mx, off, 64));
-    if (lane == 0) scales[r] = Fp8Scale(mx);
+    if (lane == 0) scales[r] = dtype == FRK_DTYPE_F16X2 ? SplitScale(mx) : Fp8Scale(mx);
 }
 
 __global__ __launch_bounds__(256) void TileKernel(const float* __restrict__ rows, uint64_t n, uint32_t dim,
@@ -23,12 +23,13 @@ __global__ __launch_bounds__(256) void TileKernel(const float* __restrict__ rows
                                                   uint64_t n_units) {
     const uint64_t u = static_cast<uint64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
     if (u >= n_units) return;
-    const uint32_t E = dtype == FRK_DTYPE_F32 ? 4 : (dtype == FRK_DTYPE_BF16 ? 8 : 16);
+    const uint32_t E = dtype == FRK_DTYPE_F32 ? 4 : (dtype == FRK_DTYPE_FP8E4M3 ? 16 : 8);
     const uint64_t b = first / 32 + u / (static_cast<uint64_t>(chunks) * 64);
     const uint32_t c = static_cast<uint32_t>((u / 64) % chunks);
     const uint32_t l = static_cast<uint32_t>(u % 64);
     const uint64_t doc = b * 32 + (l & 31);
-    const uint32_t d0 = c * 2 * E + (l >> 5) * E;
+    // F16X2: chunk pair c / 2 covers 16 dims, the hi halves in the even chunk and the lo halves in the odd
+    const uint32_t d0 = dtype == FRK_DTYPE_F16X2 ? (c / 2) * 16 + (l >> 5) * 8 : c * 2 * E + (l >> 5) * E;
     float v[16];
     const bool have = doc >= first && doc < first + n;
     const float* row = rows + (doc - first) * dim;
@@ -39,6 +40,15 @@ __global__ __launch_bounds__(256) void TileKernel(const float* __restrict__ rows
     } else if (dtype == FRK_DTYPE_BF16) {
         for (int e = 0; e < 4; e++)
             w[e] = static_cast<uint32_t>(F32ToBf16(v[2 * e])) | (static_cast<uint32_t>(F32ToBf16(v[2 * e + 1])) << 16);
+    } else if (dtype == FRK_DTYPE_F16X2) {
+        const float s = have ? scales[doc - first] : 1.0f;
+        uint16_t half[8];
+        for (int e = 0; e < 8; e++) {
+            uint16_t hi, lo;
+            SplitF16(v[e] * s, hi, lo);
+            half[e] = (c & 1) ? lo : hi;
+        }
+        for (int e = 0; e < 4; e++) w[e] = static_cast<uint32_t>(half[2 * e]) | (static_cast<uint32_t>(half[2 * e + 1]) << 16);
     } else {
         const float inv = have ? 1.0f / scales[doc - first] : 0.0f;
         for (int e = 0; e < 4; e++) {
@@ -65,12 +75,13 @@ extern "C" int frk_tile_rows_device(const float* d_rows, uint64_t n_rows, uint32
     }
     if (n_rows == 0) return FRK_OK;
     hipStream_t s = static_cast<hipStream_t>(stream);
-    if (dtype == FRK_DTYPE_FP8E4M3) {
+    const bool scaled = dtype == FRK_DTYPE_FP8E4M3 || dtype == FRK_DTYPE_F16X2;
+    if (scaled) {
         if (!d_scales) {
-            frk::SetError("frk_tile_rows_device: FP8 needs a scale array");
+            frk::SetError("frk_tile_rows_device: FP8 and F16X2 need a scale array");
             return FRK_ERR_INVALID;
         }
-        hipLaunchKernelGGL(frk::RowScaleKernel, dim3((n_rows + 3) / 4), dim3(256), 0, s, d_rows, n_rows, dim,
+        hipLaunchKernelGGL(frk::RowScaleKernel, dim3((n_rows + 3) / 4), dim3(256), 0, s, d_rows, n_rows, dim, dtype,
                            d_scales + first_doc);
         FRK_HIP_CHECK(hipGetLastError());
     }
@@ -79,7 +90,7 @@ extern "C" int frk_tile_rows_device(const float* d_rows, uint64_t n_rows, uint32
     const uint64_t n_units = n_blocks * chunks * 64;
     hipLaunchKernelGGL(frk::TileKernel, dim3((n_units + 255) / 256), dim3(256), 0, s, d_rows, n_rows, dim, first_doc,
                        dtype, chunks, static_cast<uint8_t*>(d_tiles),
-                       dtype == FRK_DTYPE_FP8E4M3 ? d_scales + first_doc : nullptr, n_units);
+                       scaled ? d_scales + first_doc : nullptr, n_units);
     FRK_HIP_CHECK(hipGetLastError());
     return FRK_OK;
 }

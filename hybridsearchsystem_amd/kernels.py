@@ -9,7 +9,7 @@ import numpy as np
 
 from . import DocStore, FilterQuery, check, lib, ptr, stream_ptr
 
-FRK_DTYPE_F32, FRK_DTYPE_BF16, FRK_DTYPE_FP8E4M3 = 0, 1, 2
+FRK_DTYPE_F32, FRK_DTYPE_BF16, FRK_DTYPE_FP8E4M3, FRK_DTYPE_F16X2 = 0, 1, 2, 3
 
 
 def chunk_dims(dtype):
@@ -30,7 +30,7 @@ def tile_docs(vectors, dtype, dim_pad=None, threads=0):
     if nbytes == 0:
         raise ValueError("bad dim_pad %d for dtype %d" % (dim_pad, dtype))
     tiles = np.empty(nbytes, dtype=np.uint8)
-    scales = np.empty(n, dtype=np.float32) if dtype == FRK_DTYPE_FP8E4M3 else None
+    scales = np.empty(n, dtype=np.float32) if dtype in (FRK_DTYPE_FP8E4M3, FRK_DTYPE_F16X2) else None
     check(lib().frk_tile_docs(v.ctypes.data, n, dim, dtype, dim_pad, tiles.ctypes.data,
                               scales.ctypes.data if scales is not None else None, threads), "frk_tile_docs")
     return tiles, scales, dim_pad

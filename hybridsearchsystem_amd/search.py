@@ -12,7 +12,7 @@ import numpy as np
 from . import check, lib
 from ._capi_sigs import IndexPostingField, IndexVectorField, SearchOptions, SearchStats
 
-FRK_DTYPE_F32, FRK_DTYPE_BF16, FRK_DTYPE_FP8E4M3 = 0, 1, 2
+FRK_DTYPE_F32, FRK_DTYPE_BF16, FRK_DTYPE_FP8E4M3, FRK_DTYPE_F16X2 = 0, 1, 2, 3
 VECTOR_FILE_FP16_ZN, VECTOR_FILE_FP32_ROWS, VECTOR_FILE_BF16_ROWS, VECTOR_FILE_FP8_ROWS = 0, 1, 2, 3
 
 

@@ -97,10 +97,17 @@ int frk_check_program(const uint32_t* expr, uint32_t expr_len, uint32_t n_postin
 #define FRK_DTYPE_F32 0
 #define FRK_DTYPE_BF16 1
 #define FRK_DTYPE_FP8E4M3 2
+/* FP32 values as two FP16 halves: with s the doc's power-of-two scale (the one that puts its largest
+ * magnitude in [2^14, 2^15)), x s = hi + lo, hi = fp16(x s), lo = fp16(x s - hi), both rounded to
+ * nearest even. Chunk c holds, per lane half h, the 8 halves of dims 16 (c / 2) + 8 h + e (e < 8): the
+ * hi parts for even c, the lo parts for odd c, so a chunk pair is one 16-dim slice, 4 bytes per
+ * element (the bytes of the FP32 store). Scored with three FP16 MFMAs per slice (hi hi + hi lo +
+ * lo hi, FP32 accumulation): products within ~2^-21 of the FP32 ones, at the FP16 MFMA rate. */
+#define FRK_DTYPE_F16X2 3
 
 typedef struct {
     const void* tiles;      /* device: n_blocks * chunks * 1 KiB */
-    const float* scales;    /* device: per-doc scale (FP8 only, else NULL) */
+    const float* scales;    /* device: per-doc scale (FP8 and F16X2, else NULL) */
     uint32_t dtype;         /* FRK_DTYPE_* */
     uint32_t dim;           /* padded dim: multiple of the chunk dims */
     uint64_t n_docs;        /* valid docs; n_blocks = ceil(n_docs / 32) */
@@ -122,14 +129,15 @@ int frk_fill_u32(uint32_t* d, uint64_t n, uint32_t value, frk_stream_t stream);
 
 /* Host-side staging of a store: row-major FP32 [n_docs][dim] -> tiles (host memory of
  * frk_tiles_bytes() bytes; dims >= dim are zero-filled up to dim_pad). FP8 also writes the per-doc
- * scale max|v| / 448 and stores v / scale. No device work; n_threads 0 = all cores. */
+ * scale max|v| / 448 and stores v / scale; F16X2 writes its per-doc scale and the hi/lo halves of v * scale.
+ * No device work; n_threads 0 = all cores. */
 uint64_t frk_tiles_bytes(uint64_t n_docs, uint32_t dim_pad, uint32_t dtype);
 int frk_tile_docs(const float* src, uint64_t n_docs, uint32_t dim, uint32_t dtype, uint32_t dim_pad, void* dst_tiles,
                   float* dst_scales, uint32_t n_threads);
 
 /* Device-side staging: rows [first_doc, first_doc + n_rows) of a store from row-major FP32 device
  * rows; first_doc must be a multiple of 32, and docs of the last touched block past n_rows are
- * zero-filled. FP8 writes d_scales[first_doc + r]. Stream-ordered. */
+ * zero-filled. FP8 and F16X2 write d_scales[first_doc + r]. Stream-ordered. */
 int frk_tile_rows_device(const float* d_rows, uint64_t n_rows, uint32_t dim, uint64_t first_doc, uint32_t dtype,
                          uint32_t dim_pad, void* d_tiles, float* d_scales, frk_stream_t stream);
 
@@ -311,8 +319,9 @@ typedef struct frk_table frk_table;
 typedef struct frk_builder frk_builder;
 
 /* DataTable::LoadData (engine/index/data_table.h:20): the index directory written by the builder
- * (.meta, id/ *.idm files, per-field .vector and .posting files). dtype_mask: extra low-precision stores to
- * build, bits (1 << FRK_DTYPE_BF16) | (1 << FRK_DTYPE_FP8E4M3); the FP32 store is always built. */
+ * (.meta, id/ *.idm files, per-field .vector and .posting files). dtype_mask: extra stores to build,
+ * bits (1 << FRK_DTYPE_BF16) | (1 << FRK_DTYPE_FP8E4M3) | (1 << FRK_DTYPE_F16X2); the FP32 store is
+ * always built. */
 int frk_table_load(int32_t device, const char* dir, uint32_t dtype_mask, frk_table** out);
 void frk_table_free(frk_table* t);
 int frk_table_info(const frk_table* t, uint64_t* n_docs, uint32_t* n_segments, uint32_t* docs_per_segment);
@@ -391,7 +400,8 @@ int frk_compile_filter(const frk_table* t, const char* field, const char* expr, 
                        uint32_t* n_postings, uint32_t* op_num, int32_t* match_all);
 
 typedef struct {
-    uint32_t dtype;           /* scoring store: FRK_DTYPE_F32 (plain), _BF16 or _FP8E4M3 */
+    uint32_t dtype;           /* scoring store: FRK_DTYPE_F32 or _F16X2 (plain: FP32-accurate scores, the
+                                 k selected directly), _BF16 or _FP8E4M3 (k * m candidates rescored in FP32) */
     uint32_t rescore_factor;  /* low precision: FP32 rescoring of the top min(k * m, FRK_SELECT_MAX_K) */
     int32_t probe;            /* sampled k-th-best floor (exact pruning) */
     uint32_t probe_step;      /* 0 = automatic */

@@ -132,6 +132,48 @@ def test_fp8_scales_are_powers_of_two(L):
     assert scales[3] == 1.0 and scales[7] == 1.0 and scales[8] == 2.0
 
 
+def test_f16x2_store_is_the_split_numpy_computes(L):
+    """The F16X2 store, tiled on the host: each doc's scale is the power of two putting max|v| in
+    [2^14, 2^15), and chunk pair p holds, per lane half h, the halves of dims 16 p + 8 h + e as
+    hi = fp16(v s) (even chunk) and lo = fp16(v s - hi) (odd chunk), bit for bit what numpy's own
+    round-to-nearest-even float16 conversion gives (magnitudes over 24 decades take both halves through
+    the subnormal range, and hand-placed values hit the rounding ties, the overflow bound and zero);
+    hi + lo is within 2^-22 of v s wherever lo is a normal half."""
+    from hybridsearchsystem_amd import kernels
+    rng = np.random.default_rng(17)
+    n, dim = 70, 48
+    docs = (rng.standard_normal((n, dim)) * 10.0 ** rng.uniform(-12, 12, (n, 1))).astype(np.float32)
+    docs[3] = 0.0
+    docs[4, :] = rng.standard_normal(dim).astype(np.float32) * 1e-7  # tiny entries beside a large one
+    docs[4, 0] = 1.0
+    docs[5, :] = np.float32(2.0 ** -10) * (1 + np.arange(dim) / 1024.0).astype(np.float32)  # exact ties
+    tiles, scales, dp = kernels.tile_docs(docs, kernels.FRK_DTYPE_F16X2)
+    assert dp == 64 and tiles.size == 3 * (dp // 8) * 1024  # 6 chunks padded to the 8-chunk shape
+    mx = np.abs(docs).max(axis=1)
+    e = np.log2(scales.astype(np.float64))
+    assert np.all(e == np.round(e))
+    nz = mx > 0
+    assert np.all(mx[nz] * scales[nz] < 2.0 ** 15) and np.all(mx[nz] * scales[nz] >= 2.0 ** 14)
+    assert scales[3] == 1.0
+    v = np.zeros((n, dp), np.float32)
+    v[:, :dim] = docs * scales[:, None]  # dims 48..63 are padding: zero halves
+    hi = v.astype(np.float16)
+    lo = (v - hi.astype(np.float32)).astype(np.float16)
+    halves = tiles.view(np.uint16).reshape(-1, dp // 8, 64, 8)  # [block, chunk, lane, e]
+    for doc in range(n):
+        b, j = divmod(doc, 32)
+        for c in range(dp // 8):
+            for h in range(2):
+                dims = (c // 2) * 16 + 8 * h + np.arange(8)
+                want = (lo if c % 2 else hi)[doc, dims].view(np.uint16)
+                assert np.array_equal(halves[b, c, j + 32 * h], want), (doc, c, h)
+    tiny = np.abs(lo.astype(np.float32)) < 2.0 ** -14
+    err = np.abs(hi.astype(np.float64) + lo.astype(np.float64) - v.astype(np.float64))
+    assert np.all(err[~tiny] <= 2.0 ** -22 * np.abs(v[~tiny]) + 1e-300)
+    # the last doc block's padding docs are zero
+    assert not halves[2, :, 6:32].any() and not halves[2, :, 38:64].any()
+
+
 def test_prepared_filters_parse_once_and_reject_bad_text(L):
     """Prepared filter handles: built without a device, match-all for None/empty, errors for bad text."""
     from hybridsearchsystem_amd import FrkError, search

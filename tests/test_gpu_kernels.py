@@ -271,18 +271,19 @@ def _check_against_truth(frk, docs, qs, bits, ks, keys, oc, rel_tol=1e-5):
                 assert abs(float(d64[d] @ qs[q].astype(np.float64)) - kth) <= 2 * allowed, (q, d)
 
 
+@pytest.mark.parametrize("dtype", [0, 3])  # the FP32 store, and its F16X2 split (three FP16 MFMAs per product)
 @pytest.mark.parametrize("n,dim,nq,counts_stride", [(5000, 128, 64, 1), (4096 + 17, 256, 100, 1), (1000, 64, 7, 1),
                                                    (20000, 256, 128, 1), (20000, 256, 128, 32), (1000, 64, 7, 3),
                                                    (30000, 64, 300, 1)])  # 300 queries: three query groups
-def test_fp32_topk_matches_cpu_truth(frk, n, dim, nq, counts_stride):
-    print("fp32 topk n=%d dim=%d nq=%d counts_stride=%d" % (n, dim, nq, counts_stride), flush=True)
+def test_fp32_topk_matches_cpu_truth(frk, n, dim, nq, counts_stride, dtype):
+    print("fp32 topk dtype=%d n=%d dim=%d nq=%d counts_stride=%d" % (dtype, n, dim, nq, counts_stride), flush=True)
     rng = np.random.default_rng(n + dim + nq)
     docs, cent = _gen.mixture_corpus(rng, n, dim, 16)
     qs = _gen.mixture_queries(rng, cent, nq)
     bits = rng.random((nq, n)) < rng.uniform(0.001, 0.7, (nq, 1))
     ks = rng.choice([1, 10, 100, 1000], size=nq).astype(np.int32)
     masks = np.stack([oracle.words_from_bits(b) for b in bits])
-    keys, oc, cnt, kth = _run_topk(frk, docs, qs, masks, ks, counts_stride=counts_stride)
+    keys, oc, cnt, kth = _run_topk(frk, docs, qs, masks, ks, dtype=dtype, counts_stride=counts_stride)
     assert np.array_equal(cnt, bits.sum(axis=1))  # unfloored: every passing doc is a candidate
     _check_against_truth(frk, docs, qs, bits, ks, keys, oc)
 
@@ -294,9 +295,35 @@ def _fp8_scale(mx):
     return np.where(mx > 0, 2.0 ** e, 1.0)
 
 
+@pytest.mark.parametrize("dtype", [0, 1, 2, 3])
+def test_device_tiling_equals_host_tiling(frk, dtype):
+    """frk_tile_rows_device (the loader's path) writes the same store bytes and scales as the host
+    tiling, in two calls whose boundary is a 32-doc block and with a ragged last block."""
+    import ctypes
+    from hybridsearchsystem_amd import check, lib, stream_ptr
+    print("device tiling dtype %d" % dtype, flush=True)
+    rng = np.random.default_rng(90 + dtype)
+    n, dim = 1000 + 13, 100
+    docs = (rng.standard_normal((n, dim)) * 10.0 ** rng.uniform(-6, 6, (n, 1))).astype(np.float32)
+    tiles, scales, dp = frk.tile_docs(docs, dtype)
+    d_rows = dev(docs)
+    d_tiles = torch.full((tiles.size,), 0x5A, dtype=torch.uint8, device="cuda")
+    d_scales = torch.zeros(n, dtype=torch.float32, device="cuda") if scales is not None else None
+    for first, cnt in ((0, 512), (512, n - 512)):
+        check(lib().frk_tile_rows_device(ctypes.c_void_p(d_rows.data_ptr() + first * dim * 4), cnt, dim, first, dtype, dp,
+                                         ctypes.c_void_p(d_tiles.data_ptr()),
+                                         ctypes.c_void_p(d_scales.data_ptr()) if d_scales is not None else None,
+                                         stream_ptr()), "frk_tile_rows_device")
+    torch.cuda.synchronize()
+    assert np.array_equal(d_tiles.cpu().numpy(), tiles)
+    if scales is not None:
+        assert np.array_equal(d_scales.cpu().numpy(), scales)
+
+
 def _dequantized(x, dtype):
-    """The values a store (or staged query tile) holds, widened to float64 (FP32 stores hold x itself)."""
-    if dtype == 0:
+    """The values a store (or staged query tile) holds, widened to float64 (FP32 stores hold x itself, and
+    F16X2 stores x to within 2^-22 of each element)."""
+    if dtype in (0, 3):
         return np.asarray(x, np.float32).astype(np.float64)
     t = torch.from_numpy(np.ascontiguousarray(x, np.float32))
     if dtype == 1:
@@ -306,7 +333,8 @@ def _dequantized(x, dtype):
 
 
 @pytest.mark.parametrize("dtype,dim,nq", [(1, 128, 70), (2, 128, 70), (1, 768, 150), (2, 768, 150), (0, 384, 150),
-                                          (0, 384, 90), (2, 1536, 150), (2, 256, 128), (2, 512, 64)])
+                                          (0, 384, 90), (2, 1536, 150), (2, 256, 128), (2, 512, 64),
+                                          (3, 128, 70), (3, 384, 150), (3, 768, 150), (3, 256, 128), (3, 64, 7)])
 def test_low_precision_scores_match_dequantized_values(frk, dtype, dim, nq):
     """Every filter-passing doc comes out once with the dot product of the stored (dequantized) query and
     doc values. Doc and query magnitudes span six decades, so a per-lane scale applied to the wrong row
@@ -314,7 +342,9 @@ def test_low_precision_scores_match_dequantized_values(frk, dtype, dim, nq):
     blocks of one query group. The wider cases run the other workgroup shapes: 48 chunks per doc (BF16
     768, FP32 384, FP8 1536 dims) with three row blocks per group and query fragments staged in
     registers (150 queries: two groups, the second partly empty; 90: one), FP8 with 8-wave workgroups
-    (768 dims; 256 dims at 128 queries, the benchmark's shape; 512 dims at 64)."""
+    (768 dims; 256 dims at 128 queries, the benchmark's shape; 512 dims at 64). F16X2 is held to the
+    FP32 store's bound against the FP64 dot product of the original values: its per-doc and per-query scales bring
+    every magnitude into FP16's range, and a dropped or misplaced hi/lo product would be off by ~2^-11."""
     print("low precision scores dtype %d dim %d nq %d" % (dtype, dim, nq), flush=True)
     rng = np.random.default_rng(60 + dtype if (dim, nq) == (128, 70) else 60 + dtype + dim + nq)
     n = 3000
@@ -476,7 +506,9 @@ def test_fp32_ties_broken_by_doc_id(frk):
         assert ids.tolist() == ids64[q, :c64[q]].tolist()
 
 
-def test_probe_floor_keeps_topk_identical(frk):
+@pytest.mark.parametrize("dtype", [0, 3])
+def test_probe_floor_keeps_topk_identical(frk, dtype):
+    print("probe floor dtype %d" % dtype, flush=True)
     rng = np.random.default_rng(33)
     n, dim, nq = 40000, 128, 64
     docs, cent = _gen.mixture_corpus(rng, n, dim, 32)
@@ -484,9 +516,9 @@ def test_probe_floor_keeps_topk_identical(frk):
     bits = rng.random((nq, n)) < rng.uniform(0.01, 0.6, (nq, 1))
     masks = np.stack([oracle.words_from_bits(b) for b in bits])
     ks = rng.choice([10, 100], size=nq).astype(np.int32)
-    keys0, oc0, cnt0, _ = _run_topk(frk, docs, qs, masks, ks)
-    _, _, _, kth = _run_topk(frk, docs, qs, masks, ks, block_step=8)  # probe: every 8th block
-    keys1, oc1, cnt1, _ = _run_topk(frk, docs, qs, masks, ks, floor=kth)
+    keys0, oc0, cnt0, _ = _run_topk(frk, docs, qs, masks, ks, dtype=dtype)
+    _, _, _, kth = _run_topk(frk, docs, qs, masks, ks, dtype=dtype, block_step=8)  # probe: every 8th block
+    keys1, oc1, cnt1, _ = _run_topk(frk, docs, qs, masks, ks, dtype=dtype, floor=kth)
     assert np.array_equal(oc0, oc1)
     for q in range(nq):
         assert keys0[q, :oc0[q]].tolist() == keys1[q, :oc1[q]].tolist()
@@ -501,16 +533,19 @@ def test_candidate_overflow_is_reported(frk):
     assert np.all(cnt == 3000)  # counts keep counting past the capacity; the caller detects cnt > cap
 
 
-def test_blockmax_probe_and_kth_largest(frk):
+@pytest.mark.parametrize("dtype", [0, 3])
+def test_blockmax_probe_and_kth_largest(frk, dtype):
+    print("blockmax dtype %d" % dtype, flush=True)
     rng = np.random.default_rng(44)
     n, dim, nq, step = 30000, 128, 40, 3
     docs, cent = _gen.mixture_corpus(rng, n, dim, 16)
     qs = _gen.mixture_queries(rng, cent, nq)
     bits = rng.random((nq, n)) < rng.uniform(0.0005, 0.5, (nq, 1))
     masks = np.stack([oracle.words_from_bits(b) for b in bits])
-    tiles, scales, dp = frk.tile_docs(docs, 0)
+    tiles, scales, dp = frk.tile_docs(docs, dtype)
     t_tiles = dev(tiles)
-    store = frk.doc_store(t_tiles, None, 0, dp, n)
+    t_scales = dev(scales) if scales is not None else None
+    store = frk.doc_store(t_tiles, t_scales, dtype, dp, n)
     nb = (n + 31) // 32
     cnt = (nb + step - 1) // step
     d_bm = torch.zeros((nq, cnt), dtype=torch.float32, device="cuda")
@@ -545,7 +580,7 @@ def test_blockmax_probe_and_kth_largest(frk):
 
 
 # ----------------------------------------------------------------------------- unfused score + aggregate
-@pytest.mark.parametrize("dtype", [0, 1, 2])
+@pytest.mark.parametrize("dtype", [0, 1, 2, 3])
 def test_score_matrix_and_aggregate(frk, dtype):
     """frk_score_matrix writes every (query, doc) score (FP32: within 1e-5 * sum|q_d v_d| of FP64;
     BF16/FP8: the dot product of the stored, dequantized values) and leaves the row padding untouched;
@@ -568,7 +603,7 @@ def test_score_matrix_and_aggregate(frk, dtype):
     torch.cuda.synchronize()
     got = d_s.cpu().numpy()
     assert np.all(got[:, n:] == 7.0)
-    if dtype == 0:
+    if dtype in (0, 3):
         ref = docs.astype(np.float64) @ qs.astype(np.float64).T
         scale = np.abs(docs.astype(np.float64)) @ np.abs(qs.astype(np.float64)).T
         assert np.all(np.abs(got[:, :n].T - ref) <= 1e-5 * scale + 1e-30)

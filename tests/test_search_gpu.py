@@ -71,17 +71,19 @@ def _check(docs, queries, masks, ks, ids, scores, counts, truth, gmap=None):
     return bad
 
 
-def test_c1_plain_fp32_matches_cpu_truth(search_mod):
-    """C1 shape: 131,072 docs x 128, one segment, 64 log-uniform tags, 256 queries, k=100."""
+@pytest.mark.parametrize("dtype", [0, 3])
+def test_c1_plain_fp32_matches_cpu_truth(search_mod, dtype):
+    """C1 shape: 131,072 docs x 128, one segment, 64 log-uniform tags, 256 queries, k=100; the plain
+    method on the FP32 store and on its F16X2 split, both held to the FP32 tolerance."""
     cfg = synth.CONFIGS["c1"]
-    print("c1 end-to-end", flush=True)
+    print("c1 end-to-end dtype %d" % dtype, flush=True)
     docs, dens, off, ids, words = _corpus(20261203, cfg["n_docs"], cfg["dim"], cfg["n_tags"], centroids=64)
     qs = synth.mixture_queries(1, 256, cfg["dim"], 64, corpus_seed=20261203)
     filters, sels = synth.random_filters(2, dens, 256, depth=3, sel=cfg["sel"])
     ks = np.full(256, 100, np.uint32)
     gd = (np.random.default_rng(9).permutation(cfg["n_docs"]).astype(np.uint64) * 7 + 11)
-    t = search_mod.Table.build(docs, off, ids, docs_per_segment=131072, gdocids=gd)
-    r_ids, r_sc, r_cnt, r_pass, st = t.search(qs, filters, ks)
+    t = search_mod.Table.build(docs, off, ids, docs_per_segment=131072, gdocids=gd, dtypes=(dtype,) if dtype else ())
+    r_ids, r_sc, r_cnt, r_pass, st = t.search(qs, filters, ks, dtype=dtype)
     masks, *truth = _truth(docs, words, qs, filters, ks)
     assert np.array_equal(r_pass, np.array([int(oracle.bits_from_words(m, len(docs)).sum()) for m in masks]))
     order = np.argsort(gd)
@@ -107,26 +109,28 @@ def test_multi_segment_all_codecs(search_mod):
         t.free()
 
 
-def test_probe_and_floors_do_not_change_results(search_mod):
+@pytest.mark.parametrize("dtype", [0, 3])
+def test_probe_and_floors_do_not_change_results(search_mod, dtype):
+    print("probe and floors dtype %d" % dtype, flush=True)
     n, dim, n_tags = 400_000, 128, 40
     docs, dens, off, ids, words = _corpus(11, n, dim, n_tags)
     qs = synth.mixture_queries(12, 64, dim, 32, corpus_seed=11)
     filters, _ = synth.random_filters(13, dens, 64, depth=3, sel=(0.001, 0.6))
     ks = np.random.default_rng(14).choice([10, 100, 1000], size=64).astype(np.uint32)
-    t = search_mod.Table.build(docs, off, ids, docs_per_segment=65536)
-    base = t.search(qs, filters, ks, probe=False)
-    probed = t.search(qs, filters, ks, probe=True, probe_step=8)
+    t = search_mod.Table.build(docs, off, ids, docs_per_segment=65536, dtypes=(dtype,) if dtype else ())
+    base = t.search(qs, filters, ks, probe=False, dtype=dtype)
+    probed = t.search(qs, filters, ks, probe=True, probe_step=8, dtype=dtype)
     assert probed[4]["probe_step"] == 8
     assert probed[4]["candidates"] < base[4]["candidates"]
     assert np.array_equal(base[0], probed[0]) and np.array_equal(base[1], probed[1])
     # a caller floor above the k-th score must be detected and the batch rerun unpruned
     kth = np.array([base[1][q, base[2][q] - 1] if base[2][q] else -np.inf for q in range(64)], np.float32)
     high = kth + 0.05
-    floored = t.search(qs, filters, ks, floors=high)
+    floored = t.search(qs, filters, ks, floors=high, dtype=dtype)
     assert floored[4]["reruns"] > 0
     assert np.array_equal(base[0], floored[0]) and np.array_equal(base[1], floored[1])
     low = kth - 0.01
-    safe = t.search(qs, filters, ks, floors=low)
+    safe = t.search(qs, filters, ks, floors=low, dtype=dtype)
     assert safe[4]["reruns"] == 0
     assert np.array_equal(base[0], safe[0]) and np.array_equal(base[1], safe[1])
 
@@ -171,20 +175,22 @@ def test_low_precision_with_fp32_rescoring(search_mod, dtype, min_recall):
 @pytest.mark.parametrize("dim", [24, 72, 96, 100, 192, 384, 768])
 def test_any_dimension(search_mod, dim):
     """Dims that are not a compiled chunk count are zero-padded to the next one (12- and 24-chunk
-    shapes cover 96/192/384/768-dim stores without padding): plain FP32 against the CPU truth, BF16 and
+    shapes cover 96/192/384/768-dim stores without padding): plain FP32 and F16X2 against the CPU truth, BF16 and
     FP8 with FP32 rescoring of the top k*4 (FP8 also k*8) at recall@k >= 0.99 against it, over 100 queries (4 row blocks per workgroup where the
     query tile fits)."""
     from hybridsearchsystem_amd import kernels
     n, n_tags, nq = 8000 + 45, 20, 100
-    print("dim %d: store dims %s" % (dim, [kernels.padded_dim(dim, d) for d in range(3)]), flush=True)
+    print("dim %d: store dims %s" % (dim, [kernels.padded_dim(dim, d) for d in range(4)]), flush=True)
     docs, dens, off, ids, words = _corpus(40 + dim, n, dim, n_tags)
     qs = synth.mixture_queries(41 + dim, nq, dim, 32, corpus_seed=40 + dim)
     filters, _ = synth.random_filters(42 + dim, dens, nq - 1, depth=2, sel=(0.01, 0.9))
     filters = [None] + filters
     ks = np.random.default_rng(dim).choice([1, 10, 50], size=nq).astype(np.uint32)
-    t = search_mod.Table.build(docs, off, ids, docs_per_segment=4096, dtypes=(1, 2))
+    t = search_mod.Table.build(docs, off, ids, docs_per_segment=4096, dtypes=(1, 2, 3))
     masks, *truth = _truth(docs, words, qs, filters, ks)
     r = t.search(qs, filters, ks)
+    assert _check(docs, qs, masks, ks, r[0], r[1], r[2], truth) <= 2
+    r = t.search(qs, filters, ks, dtype=3)
     assert _check(docs, qs, masks, ks, r[0], r[1], r[2], truth) <= 2
     ids32, _, cnt32 = truth
     for dtype, m in ((1, 4), (2, 4), (2, 8)):
@@ -266,8 +272,8 @@ def test_c2_plain_fp32_5M_docs(search_mod):
 
 def test_edge_cases_every_store(search_mod):
     """Filters that pass nothing (a term absent from the index, a contradiction), fewer passing docs
-    than k (k = 4096 over a rare tag), k = 1, match-all, and a batch of one query, on the FP32, BF16 and
-    FP8 paths: counts equal min(k, passing), an empty result is an empty row, and where every passing
+    than k (k = 4096 over a rare tag), k = 1, match-all, and a batch of one query, on the FP32, F16X2,
+    BF16 and FP8 paths: counts equal min(k, passing), an empty result is an empty row, and where every passing
     doc fits in k each store returns exactly the passing set."""
     n, dim, n_tags = 20000 + 13, 64, 12
     docs, dens, off, ids, words = _corpus(31, n, dim, n_tags)
@@ -280,12 +286,16 @@ def test_edge_cases_every_store(search_mod):
     passing = np.array([int(oracle.bits_from_words(m, n).sum()) for m in masks])
     assert passing[0] == 0 and passing[1] == 0 and 0 < passing[2] < 4096
     t = search_mod.Table.build(docs, off, ids, docs_per_segment=8192,
-                               dtypes=(search_mod.FRK_DTYPE_BF16, search_mod.FRK_DTYPE_FP8E4M3))
+                               dtypes=(search_mod.FRK_DTYPE_BF16, search_mod.FRK_DTYPE_FP8E4M3,
+                                       search_mod.FRK_DTYPE_F16X2))
     plain = t.search(qs, filters, ks)
     assert np.array_equal(plain[3], passing)
     assert _check(docs, qs, masks, ks, plain[0], plain[1], plain[2], truth) <= 1
     for q in (0, 1):
         assert plain[2][q] == 0 and np.all(plain[0][q] == -1)
+    split = t.search(qs, filters, ks, dtype=search_mod.FRK_DTYPE_F16X2)
+    assert np.array_equal(split[3], passing)
+    assert _check(docs, qs, masks, ks, split[0], split[1], split[2], truth) <= 1
     for dtype in (search_mod.FRK_DTYPE_BF16, search_mod.FRK_DTYPE_FP8E4M3):
         print("edge cases dtype", dtype, flush=True)
         r = t.search(qs, filters, ks, dtype=dtype, rescore_factor=4)
