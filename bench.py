@@ -263,8 +263,8 @@ def measure_hbm(n_bytes=4 << 30, reps=10):
     return out
 
 
-DTYPE_NAMES = {0: "f32", 1: "bf16", 2: "fp8"}
-ELEM_BYTES = {0: 4, 1: 2, 2: 1}
+DTYPE_NAMES = {0: "f32", 1: "bf16", 2: "fp8", 3: "f16x2"}
+ELEM_BYTES = {0: 4, 1: 2, 2: 1, 3: 4}
 
 
 def timed_leg(table, queries, filters, ks, args, res_p, name, dtype, streams=1, steps=None, **kw):
@@ -386,9 +386,9 @@ def learned_leg(table, args, streams, res_p, centroids, seed, legs):
 
 def kernel_roofline(stats, n, d, q, dtype):
     """Roofline of the main fused score pass: algorithmic bytes per launch (the doc stream once, the per-doc
-    filter words, FP8 scales, the 8-byte candidate keys written) over its HIP-event time."""
+    filter words, FP8 / F16X2 doc scales, the 8-byte candidate keys written) over its HIP-event time."""
     n_blocks = (n + 31) // 32
-    doc_bytes = n_blocks * 32 * d * ELEM_BYTES[dtype] + (n_blocks * 32 * 4 if dtype == 2 else 0)
+    doc_bytes = n_blocks * 32 * d * ELEM_BYTES[dtype] + (n_blocks * 32 * 4 if dtype in (2, 3) else 0)
     mask_bytes = n_blocks * ((q + 31) // 32 * 32) * 4
     cand_bytes = float(np.mean([s["candidates"] for s in stats])) * 8
     alg = doc_bytes + mask_bytes + cand_bytes
@@ -398,43 +398,57 @@ def kernel_roofline(stats, n, d, q, dtype):
 
 def leg_c2(args, threads):
     """The plain FP32 GPU configuration: 5M x 256 in 39 segments of 131,072, 1,024 Zipf tags, 4,096
-    queries in batches of 64 with k in {10, 100, 1000} and selectivity 0.01%-60%; GPU plain FP32 timed
-    over all 64 batches, its score kernel against the FP32 MFMA peak; the reference CPU brute force on
-    one full batch, and recall@k against it."""
+    queries in batches of 64 with k in {10, 100, 1000} and selectivity 0.01%-60%; the plain method timed
+    over all 64 batches on the FP32 store (score kernel against the FP32 MFMA peak) and on its F16X2
+    split (against the HBM peak); the reference CPU brute force on one full batch, and recall@k against
+    it."""
     import torch
     from hybridsearchsystem_amd import synth
-    from hybridsearchsystem_amd.search import FRK_DTYPE_F32
+    from hybridsearchsystem_amd.search import FRK_DTYPE_F16X2, FRK_DTYPE_F32
     cfg = synth.CONFIGS["c2"]
     n, dim, bq = cfg["n_docs"], cfg["dim"], cfg["batch"]
     S = synth.SEEDS
     dens = synth.zipf_densities(cfg["n_tags"])
     off, ids = synth.gen_postings(np.random.default_rng(S["tags"]), n, dens)
-    table, head = build_table(n, dim, cfg["centroids"], (dens, off, ids), (), S["corpus"], keep_host=not args.no_cpu)
+    table, head = build_table(n, dim, cfg["centroids"], (dens, off, ids), (FRK_DTYPE_F16X2,), S["corpus"],
+                              keep_host=not args.no_cpu)
     qs = synth.mixture_queries(S["queries"], cfg["n_queries"], dim, cfg["centroids"], corpus_seed=S["corpus"])
     fs, _ = synth.random_filters(S["filters"], dens, cfg["n_queries"], depth=cfg["depth"], sel=cfg["sel"])
     ks = synth.draw_ks(S["kdraw"], cfg["n_queries"], cfg["ks"])
     pf = table.prepare_filters(fs)
     steps = cfg["n_queries"] // bq
-    kw = dict(dtype=FRK_DTYPE_F32, graph=not args.eager)
-    run_steps(table, qs, pf, ks, 3, bq, **kw)
-    torch.cuda.synchronize()
-    t0 = time.perf_counter()
-    lat, st, res = run_steps(table, qs, pf, ks, steps, bq, **kw)
-    torch.cuda.synchronize()
-    wall = time.perf_counter() - t0
-    _, st_e, _ = run_steps(table, qs, pf, ks, 8, bq, dtype=FRK_DTYPE_F32, graph=False)
-    ms = float(np.mean([s["ms_score"] for s in st_e]))
-    tf = 2.0 * bq * n * dim / (ms * 1e-3) / 1e12
-    gbs = n * dim * 4 / (ms * 1e-3) / 1e9
-    out = {"workload": "5M x 256 plain FP32, 39 segments, batches of 64, k in {10,100,1000}, 1,024 Zipf tags",
-           "qps": round(cfg["n_queries"] / wall, 2), "p50_ms": round(float(np.percentile(lat, 50)), 3),
-           "p99_ms": round(float(np.percentile(lat, 99)), 3), "steps": steps,
-           "stages_ms": {k: round(float(np.mean([s[k] for s in st_e])), 3) for k in
-                         ("ms_filter", "ms_probe", "ms_score", "ms_select", "ms_total")},
-           "probe_step": int(st_e[-1]["probe_step"]),
-           "roofline": {"bound": "mfma", "achieved": round(tf, 1), "peak": MFMA_PEAK_TFLOPS["f32"], "unit": "TFLOP/s",
-                        "frac": round(tf / MFMA_PEAK_TFLOPS["f32"], 4), "doc_stream_gbs": round(gbs, 1),
-                        "doc_stream_frac_of_hbm_peak": round(gbs / HBM_PEAK_GBS, 4), "kernel_ms": round(ms, 4)}}
+    out = {"workload": "5M x 256 plain method, 39 segments, batches of 64, k in {10,100,1000}, 1,024 Zipf tags"}
+    res = None
+    for name, dtype in (("fp32_mfma", FRK_DTYPE_F32), ("f16x2", FRK_DTYPE_F16X2)):
+        kw = dict(dtype=dtype, graph=not args.eager)
+        run_steps(table, qs, pf, ks, 3, bq, **kw)
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        lat, st, r = run_steps(table, qs, pf, ks, steps, bq, **kw)
+        torch.cuda.synchronize()
+        wall = time.perf_counter() - t0
+        res = res or r
+        _, st_e, _ = run_steps(table, qs, pf, ks, 8, bq, dtype=dtype, graph=False)
+        ms = float(np.mean([x["ms_score"] for x in st_e]))
+        leg = {"qps": round(cfg["n_queries"] / wall, 2), "p50_ms": round(float(np.percentile(lat, 50)), 3),
+               "p99_ms": round(float(np.percentile(lat, 99)), 3), "steps": steps,
+               "stages_ms": {k: round(float(np.mean([x[k] for x in st_e])), 3) for k in
+                             ("ms_filter", "ms_probe", "ms_score", "ms_select", "ms_total")},
+               "probe_step": int(st_e[-1]["probe_step"])}
+        if dtype == FRK_DTYPE_F32:
+            tf = 2.0 * bq * n * dim / (ms * 1e-3) / 1e12
+            gbs = n * dim * 4 / (ms * 1e-3) / 1e9
+            leg["roofline"] = {"bound": "mfma", "achieved": round(tf, 1), "peak": MFMA_PEAK_TFLOPS["f32"],
+                               "unit": "TFLOP/s", "frac": round(tf / MFMA_PEAK_TFLOPS["f32"], 4),
+                               "doc_stream_gbs": round(gbs, 1), "doc_stream_frac_of_hbm_peak": round(gbs / HBM_PEAK_GBS, 4),
+                               "kernel_ms": round(ms, 4)}
+        else:
+            alg, ms, gbs = kernel_roofline(st_e, n, dim, bq, dtype)
+            leg["roofline"] = {"bound": "hbm", "achieved": round(gbs, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                               "frac": round(gbs / HBM_PEAK_GBS, 4), "alg_bytes": int(alg), "kernel_ms": round(ms, 4)}
+            leg["recall_at_k_vs_fp32_mfma"] = round(recall(r, res), 5)
+        out[name] = leg
+        log("c2 %s: %.0f q/s, score kernel %.3f ms" % (name, leg["qps"], ms))
     if not args.no_cpu:
         words = synth.doc_tag_words(off, ids, n, cfg["n_tags"])
         rep, truth = cpu_baseline(head, words, qs[:bq], fs[:bq], ks[:bq], threads, bq)
@@ -521,15 +535,15 @@ def main():
         sys.exit(2)
     import torch
     from hybridsearchsystem_amd import synth
-    from hybridsearchsystem_amd.search import FRK_DTYPE_BF16, FRK_DTYPE_F32, FRK_DTYPE_FP8E4M3
+    from hybridsearchsystem_amd.search import FRK_DTYPE_BF16, FRK_DTYPE_F16X2, FRK_DTYPE_F32, FRK_DTYPE_FP8E4M3
 
     threads = args.cpu_threads or os.cpu_count() or 1
     S = synth.SEEDS
     centroids, seed = 1024, S["corpus"]
     dens, off, ids = gen_tags(args.docs, S["tags"])
     args.dens_l, args.dens_z = dens[:N_LOGU_TAGS], dens[N_LOGU_TAGS:]
-    table, head = build_table(args.docs, args.dim, centroids, (dens, off, ids), (FRK_DTYPE_BF16, FRK_DTYPE_FP8E4M3),
-                              seed, keep_host=not args.no_cpu)
+    table, head = build_table(args.docs, args.dim, centroids, (dens, off, ids),
+                              (FRK_DTYPE_BF16, FRK_DTYPE_FP8E4M3, FRK_DTYPE_F16X2), seed, keep_host=not args.no_cpu)
     queries = synth.mixture_queries(S["queries"], args.queries, args.dim, centroids, corpus_seed=seed)
     f_c3, _ = synth.random_filters(S["filters"], args.dens_l, args.queries, depth=3, sel=(0.001, 0.5))
     zq = synth.mixture_queries(S["queries"] + 100, args.queries, args.dim, centroids, corpus_seed=seed)
@@ -544,6 +558,11 @@ def main():
     plain = timed_leg(table, qs, pf, ks, args, None, "plain fp32", FRK_DTYPE_F32, steps=args.steps)
     res_p = plain["res"]
     _, st_pe, _ = run_steps(table, qs, pf, ks, args.select_steps, args.batch, dtype=FRK_DTYPE_F32, graph=False)
+    # the same plain method on the F16X2 split of the FP32 store (FP32-accurate scores from FP16 MFMAs)
+    plain_x = timed_leg(table, qs, pf, ks, args, res_p, "plain f16x2", FRK_DTYPE_F16X2, steps=args.steps)
+    _, st_xe, _ = run_steps(table, qs, pf, ks, args.select_steps, args.batch, dtype=FRK_DTYPE_F16X2, graph=False)
+    alg_x, ms_x, gbs_x = kernel_roofline(st_xe, args.docs, args.dim, args.batch, FRK_DTYPE_F16X2)
+    fastest_plain = max(plain, plain_x, key=lambda r: r["qps"])
     # low-precision legs: (store, rescoring factor m, batch streams); FP32 rescoring of the top k*m
     # recovers recall. m in {2, 4, 8} for both stores with the probe floor (the per-m precision report);
     # the fast ones also with the learned floor and with two or three batches in flight.
@@ -603,9 +622,16 @@ def main():
         "plain_fp32": dict(pl(plain), score_kernel_ms=round(ms_score_p, 3),
                            roofline={"bound": "mfma", "achieved": round(tf_p, 1), "peak": MFMA_PEAK_TFLOPS["f32"],
                                      "unit": "TFLOP/s", "frac": round(tf_p / MFMA_PEAK_TFLOPS["f32"], 4)}),
+        "plain_f16x2": dict(pl(plain_x), score_kernel_ms=round(ms_x, 3),
+                            roofline={"bound": "hbm", "achieved": round(gbs_x, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                                      "frac": round(gbs_x / HBM_PEAK_GBS, 4), "alg_bytes": int(alg_x)}),
+        "plain_note": "the plain method (FP32-accurate scores, the k selected directly) on the FP32 store with FP32 "
+                      "MFMAs and on its F16X2 split with three FP16 MFMAs per product; speedup_vs_plain divides by the "
+                      "faster of the two",
         "precision_by_m": precision,
         "chosen": {"qps": round(chosen["qps"], 2), "pipeline": chosen["name"],
-                   "speedup_vs_plain": round(chosen["qps"] / plain["qps"], 3),
+                   "speedup_vs_plain": round(chosen["qps"] / fastest_plain["qps"], 3),
+                   "speedup_vs_plain_fp32_mfma": round(chosen["qps"] / plain["qps"], 3),
                    "selection": {r["name"]: {"qps": round(r["qps"], 2), "recall": round(r["recall"], 5),
                                              "p99_ms": round(float(np.percentile(r["lat"], 99)), 3)} for r in runs},
                    "selection_note": "each candidate timed over %d batches; the chosen one re-timed alone over %d" % (
@@ -624,6 +650,15 @@ def main():
     }
     if learned_out is not None:
         out["learned"] = learned_out
+        # the target's pipeline: BF16 store + learned floor + FP32 rescoring, against both plain paths
+        bl = [r for r in runs if r["name"].startswith("bf16 + learned")]
+        if bl:
+            b = max(bl, key=lambda r: r["qps"])
+            out["target_bf16_learned"] = {
+                "pipeline": b["name"],
+                "qps": round(b["qps"], 2), "recall_at_k_vs_plain": round(b["recall"], 5),
+                "vs_plain_fp32_mfma": round(b["qps"] / plain["qps"], 3),
+                "vs_plain_f16x2": round(b["qps"] / plain_x["qps"], 3)}
     if not args.no_cpu:
         nq = min(args.cpu_queries, args.queries)
         words = synth.doc_tag_words(off, ids, args.docs, N_LOGU_TAGS)  # the headline filters use tags 0..63
