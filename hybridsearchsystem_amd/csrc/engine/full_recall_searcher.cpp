@@ -643,13 +643,16 @@ bool FullRecallSearcher::RunDevice(const VectorFieldData& vf, std::vector<AggrDe
 
     // ---- device buffers
     const uint64_t nBlocks = (static_cast<uint64_t>(nDocs) + 31) / 32;
-    // Probe sample: every probeStep-th doc block, reduced to one max per (query, block). Automatic step:
-    // keep >= 64 sampled blocks per selected key so the floor is tight, sample at most every 64th block
-    // and at least every 2nd (a batch with k * m in the thousands then still gets a floor from half the
-    // blocks, instead of scoring every passing doc into its candidate lists).
+    // Probe sample: every probeStep-th doc block, reduced to one max per (query, block). The floor is the
+    // k'-th largest sampled block maximum, about the (k' * step)-th best passing score, so the main pass
+    // keeps ~k' * step survivors per query while the probe reads 1 / step of the store. Automatic step:
+    // ~8,192 survivors for the batch's largest k' (a survivor costs the main pass far less than a
+    // sampled block costs the probe), at least 4 sampled blocks per key, every 2nd to every 64th block.
     uint32_t probeStep = m_options.probeStep;
     const bool autoStep = probeStep == 0;
-    if (autoStep) probeStep = static_cast<uint32_t>(std::min<uint64_t>(64, std::max<uint64_t>(2, nBlocks / (64ull * kSel))));
+    if (autoStep)
+        probeStep = static_cast<uint32_t>(std::min<uint64_t>(
+            64, std::max<uint64_t>(2, std::min<uint64_t>(8192 / kSel, nBlocks / (4ull * kSel)))));
     // a query whose k*m exceeds the sampled blocks gets a -inf floor (no pruning), which stays exact
     uint32_t kSelMin = kSel;
     for (uint32_t k : m_topKs) kSelMin = std::min(kSelMin, kPrime(k));

@@ -229,16 +229,48 @@ __global__ __launch_bounds__(kThreads) void SelectKernel(const uint64_t* __restr
     }
 }
 
-// k-th largest of n floats per query (radix select over order-preserving u32 keys, 4 passes, over an
-// LDS copy of the values when they fit).
+// One radix pass's verdict over a 2048-bin histogram, by the first wave: lane l sums bins
+// 2047 - 32 l .. 2016 - 32 l, a wave prefix sum finds the lane whose bins cross rem, and that lane walks
+// its 32 bins. Needs rem <= the histogram's total.
+__device__ inline void FindDigit2048(const uint32_t* hist, uint32_t rem, uint32_t* digit, uint32_t* above) {
+    const int lane = threadIdx.x & 63;
+    uint32_t sum = 0;
+    for (int b = 0; b < 32; b++) sum += hist[2047 - 32 * lane - b];
+    uint32_t incl = sum;
+#pragma unroll
+    for (int off = 1; off < 64; off <<= 1) {
+        const uint32_t o = __shfl_up(incl, off, 64);
+        if (lane >= off) incl += o;
+    }
+    const uint32_t excl = incl - sum;
+    if (excl < rem && rem <= incl) {
+        uint32_t cum = excl;
+        int b = 0;
+        for (; b < 31; b++) {
+            const uint32_t c = hist[2047 - 32 * lane - b];
+            if (cum + c >= rem) break;
+            cum += c;
+        }
+        *digit = static_cast<uint32_t>(2047 - 32 * lane - b);
+        *above = cum;
+    }
+}
+
+// k-th largest of n floats per query: an MSB-first radix select over order-preserving u32 keys in three
+// passes of 11, 11 and 10 bits, one workgroup per query. Histograms are 2048 bins in LDS, split over
+// four sub-histograms (each shared by a quarter of the waves) to spread the atomics; the values are kept
+// in LDS when they fit, and otherwise each pass reads them with eight independent loads per thread in
+// flight (the loop is bound by load latency, not bandwidth).
+constexpr int kKthSub = 4;
+constexpr int kKthUnroll = 8;
+
 __global__ __launch_bounds__(kThreads) void KthLargestKernel(const float* __restrict__ vals, uint32_t stride,
                                                              uint32_t n, const uint32_t* __restrict__ ks,
                                                              uint32_t k_default, float* __restrict__ out) {
-    __shared__ uint32_t hist[256];
-    __shared__ uint32_t whist[256 * kWaves];  // per-wave sub-histograms
+    __shared__ uint32_t hist[kKthSub * 2048];
     __shared__ uint32_t s_digit, s_above;
     __shared__ uint32_t okeys[kLdsVals];
-    const int wave = threadIdx.x >> 6;
+    const uint32_t sub = (threadIdx.x >> 6) % kKthSub;
     const uint32_t q = blockIdx.x;
     const uint32_t k = ks ? ks[q] : k_default;
     const float* v = vals + static_cast<uint64_t>(q) * stride;
@@ -250,27 +282,46 @@ __global__ __launch_bounds__(kThreads) void KthLargestKernel(const float* __rest
     if (in_lds)
         for (uint32_t i = threadIdx.x; i < n; i += kThreads) okeys[i] = OrderedFromFloat(v[i]);
     uint32_t prefix = 0, mask = 0, rem = k;
-    for (int shift = 24; shift >= 0; shift -= 8) {
-        for (int b = threadIdx.x; b < 256 * kWaves; b += kThreads) whist[b] = 0;
+    constexpr int kShift[3] = {21, 10, 0};
+    constexpr uint32_t kBins[3] = {2048u, 2048u, 1024u};
+#pragma unroll
+    for (int pass = 0; pass < 3; pass++) {
+        const int shift = kShift[pass];
+        const uint32_t bin_mask = kBins[pass] - 1u;
+        for (int b = threadIdx.x; b < kKthSub * 2048; b += kThreads) hist[b] = 0;
         __syncthreads();
         if (threadIdx.x == 0) s_digit = s_above = 0;
-        for (uint32_t i = threadIdx.x; i < n; i += kThreads) {
-            const uint32_t key = in_lds ? okeys[i] : OrderedFromFloat(v[i]);
-            if ((key & mask) == prefix) atomicAdd(&whist[wave * 256 + ((key >> shift) & 255u)], 1u);
+        auto count = [&](uint32_t key) {
+            if ((key & mask) == prefix) atomicAdd(&hist[sub * 2048 + ((key >> shift) & bin_mask)], 1u);
+        };
+        if (in_lds) {
+            for (uint32_t i = threadIdx.x; i < n; i += kThreads) count(okeys[i]);
+        } else {
+            for (uint32_t base = 0; base < n; base += kThreads * kKthUnroll) {
+                float x[kKthUnroll];
+#pragma unroll
+                for (int u = 0; u < kKthUnroll; u++) {
+                    const uint32_t i = base + u * kThreads + threadIdx.x;
+                    x[u] = i < n ? v[i] : 0.0f;
+                }
+#pragma unroll
+                for (int u = 0; u < kKthUnroll; u++)
+                    if (base + u * kThreads + threadIdx.x < n) count(OrderedFromFloat(x[u]));
+            }
         }
         __syncthreads();
-        for (int b = threadIdx.x; b < 256; b += kThreads) {
+        for (int b = threadIdx.x; b < 2048; b += kThreads) {
             uint32_t t = 0;
 #pragma unroll
-            for (int w = 0; w < kWaves; w++) t += whist[w * 256 + b];
-            hist[b] = t;
+            for (int w = 0; w < kKthSub; w++) t += hist[w * 2048 + b];
+            hist[b] = t;  // sub-histogram 0 becomes the total (its bins are read before written, by this thread)
         }
         __syncthreads();
-        if (threadIdx.x < 64) FindDigit(hist, rem, &s_digit, &s_above);
+        if (threadIdx.x < 64) FindDigit2048(hist, rem, &s_digit, &s_above);
         __syncthreads();
         prefix |= s_digit << shift;
         rem -= s_above;
-        mask |= 0xFFu << shift;
+        mask |= bin_mask << shift;
     }
     if (threadIdx.x == 0) out[q] = FloatFromOrdered(prefix);
 }

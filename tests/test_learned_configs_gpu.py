@@ -106,7 +106,7 @@ def test_c4_learned_floor_on_low_precision_store(c4, dtype):
     assert learned[4]["reruns"] <= 0.1 * len(qs)
     # what the network gains over the same pipeline without it: survivors written by the main pass, per
     # k; beside it the exact probe floor, which buys its floor with a sampled scoring pass over the
-    # store (every 2nd block at this corpus size) where the network costs one small MLP per batch
+    # store (every step-th block) where the network costs one small MLP per batch
     total = {"none": 0, "learned": 0}
     for k in (10, 100, 1000):
         sel = np.nonzero(ks == k)[0]
