@@ -350,7 +350,8 @@ def learned_leg(table, args, streams, res_p, centroids, seed, legs):
             "p99_ms": round(float(np.percentile(r["lat"], 99)), 3), "recall_at_k_vs_plain": round(r["recall"], 5),
             "candidates_per_query": round(float(np.mean([s["candidates"] for s in r["stats"]])) / args.batch, 1),
             "reruns_per_batch": round(float(np.mean([s["reruns"] for s in r["stats"]])), 3),
-            "device_ms_per_batch": round(float(np.mean([s["ms_total"] for s in r["stats"]])), 3)}
+            "device_ms_per_batch": round(float(np.mean([s["ms_total"] for s in r["stats"]])), 3),
+            "host_prepare_ms_per_batch": round(float(np.mean([s["ms_host_prepare"] for s in r["stats"]])), 3)}
     # held-out evaluation
     n = args.learned_eval
     eq, ef, ek = mixed(S["eval_queries"], S["eval_filters"], S["eval_kdraw"], n)
@@ -641,6 +642,7 @@ def main():
                         "speedup_vs_plain": round(z_best["qps"] / z_plain["qps"], 3)},
         "graph_replay": not args.eager,
         "device_ms_per_batch": round(float(np.mean([s["ms_total"] for s in chosen["stats"]])), 3),
+        "host_prepare_ms_per_batch": round(float(np.mean([s["ms_host_prepare"] for s in chosen["stats"]])), 3),
         "stages_ms": {k: round(float(np.mean([s[k] for s in st_e])), 3) for k in
                       ("ms_filter", "ms_probe", "ms_score", "ms_select", "ms_rescore", "ms_total")},
         "stages_note": "stages_ms and roofline.kernel_ms: HIP events on the searcher stream in an eager pass of the "

@@ -116,76 +116,45 @@ int Run(const frk_doc_store* store, const ScoreLaunch& L, int mode) {
     return FRK_ERR_INVALID;
 }
 
-// Query-major filter words -> block-major 32-bit masks, through a 64 x 64 LDS tile so both the reads
-// (along a query's words) and the writes (along a block's queries) are coalesced.
-// A workgroup of 256 threads transposes the filter words of 64 doc blocks (32 u64 words per query)
-// for a tile of up to 128 queries (4 groups of 32):
-//   1. the tile's query rows are loaded into LDS with coalesced 8-byte loads (two 256-byte rows per
-//      wave instruction);
-//   2. thread (group g = t / 64, block b = t % 64) reads the 32 rows of its group for block b (a wave
-//      reads one row's 256 bytes at a time: no bank conflicts) and transposes that 32 x 32 bit matrix
-//      in registers (five rounds of off-diagonal block swaps: 16-, 8-, 4-, 2- and 1-bit blocks);
-//   3. the 32 per-doc words go back to LDS in output order (rows padded to 132 words, so 16 lanes of
-//      16-byte stores cover all banks), then the workgroup writes each block's 128 g words with
-//      contiguous 16-byte stores.
-constexpr int kMtbBlocks = 64;               // doc blocks per workgroup
-constexpr int kMtbGroups = 4;                // query groups of 32 per workgroup
-constexpr int kMtbOutPitch = 32 * kMtbGroups + 4;  // u32 words per block row of the output stage
-
-__global__ __launch_bounds__(256) void MasksToBlocksKernel(const uint64_t* __restrict__ masks, uint64_t stride,
-                                                           uint32_t n_queries, uint64_t n_blocks,
-                                                           uint32_t* __restrict__ out, uint32_t q_stride) {
-    // one 33 KiB buffer: the query rows [query][word] (32 KiB) in phases 1-2, the output stage
-    // [block][g * 32 + j] in phase 3 (so four workgroups share a CU)
-    __shared__ __align__(16) uint32_t lds[kMtbBlocks * kMtbOutPitch];
-    uint64_t (*rows)[kMtbBlocks / 2] = reinterpret_cast<uint64_t (*)[kMtbBlocks / 2]>(lds);
-    uint32_t* stage = lds;
-    const uint32_t t = threadIdx.x;
-    const uint64_t b0 = static_cast<uint64_t>(blockIdx.x) * kMtbBlocks;
-    const uint64_t w0 = b0 / 2;
-    const uint64_t n_words = (n_blocks + 1) / 2;
-    const uint32_t q0 = blockIdx.y * 32 * kMtbGroups;
-    const uint32_t groups = min(static_cast<uint32_t>(kMtbGroups), (q_stride - q0) / 32);
-    for (uint32_t e = t; e < 32 * groups * (kMtbBlocks / 2); e += 256) {
-        const uint32_t r = e / (kMtbBlocks / 2), wi = e % (kMtbBlocks / 2);
-        const uint32_t q = q0 + r;
-        const uint64_t w = w0 + wi;
-        rows[r][wi] = (q < n_queries && w < n_words) ? masks[static_cast<uint64_t>(q) * stride + w] : 0ull;
-    }
-    __syncthreads();
-    const uint32_t g = t / kMtbBlocks, b = t % kMtbBlocks;
-    uint32_t w[32];
-    if (g < groups) {
+// 32 x 32 bit transpose of w[0..31] in registers (five rounds of off-diagonal block swaps).
+__device__ __forceinline__ void Transpose32(uint32_t (&w)[32]) {
 #pragma unroll
-        for (int x = 0; x < 32; x++) w[x] = static_cast<uint32_t>(rows[32 * g + x][b / 2] >> (32 * (b & 1)));
-        // after the round of block size s, rows k and k + s (k with bit s clear) have traded the high
-        // s-bit fields of row k for the low s-bit fields of row k + s in every 2s-bit group
+    for (int s = 16; s >= 1; s >>= 1) {
+        const uint32_t m = s == 16 ? 0x0000FFFFu : s == 8 ? 0x00FF00FFu : s == 4 ? 0x0F0F0F0Fu : s == 2 ? 0x33333333u : 0x55555555u;
 #pragma unroll
-        for (int s = 16; s >= 1; s >>= 1) {
-            const uint32_t m = s == 16 ? 0x0000FFFFu : s == 8 ? 0x00FF00FFu : s == 4 ? 0x0F0F0F0Fu : s == 2 ? 0x33333333u : 0x55555555u;
-#pragma unroll
-            for (int k = 0; k < 32; k++) {
-                if (k & s) continue;
-                const uint32_t tt = ((w[k] >> s) ^ w[k + s]) & m;
-                w[k] ^= tt << s;
-                w[k + s] ^= tt;
-            }
+        for (int k = 0; k < 32; k++) {
+            if (k & s) continue;
+            const uint32_t tt = ((w[k] >> s) ^ w[k + s]) & m;
+            w[k] ^= tt << s;
+            w[k + s] ^= tt;
         }
     }
-    __syncthreads();  // the rows are read: the buffer becomes the output stage
-    if (g < groups) {
-        u32x4* dst = reinterpret_cast<u32x4*>(stage + b * kMtbOutPitch + 32 * g);
+}
+
+// Query-major filter words -> block-major 32-bit masks. Thread (query group g, block b) loads the 32-bit
+// word of block b from each of its group's 32 query rows (a wave instruction reads 64 consecutive blocks
+// of one row: 256 contiguous bytes; the 32 loads are independent and all in flight together), transposes
+// the 32 x 32 bit matrix in registers and writes the block's 32 per-doc words as one 128-byte line (8
+// 16-byte stores). No LDS and no barrier: at 10M docs and 128 queries 0.085 ms, where staging 64 blocks x
+// 128 queries through an LDS tile (coalesced loads and stores on both sides of two barriers) took 0.107
+// ms; wider loads (2 or 4 blocks per thread) measured slower still (0.13-0.14 ms).
+__global__ __launch_bounds__(256) void MasksToBlocksKernel(const uint32_t* __restrict__ masks32, uint64_t stride32,
+                                                           uint32_t n_queries, uint64_t n_blocks,
+                                                           uint32_t* __restrict__ out, uint32_t q_stride) {
+    const uint32_t t = threadIdx.x;
+    const uint64_t b = static_cast<uint64_t>(blockIdx.x) * 64 + (t & 63);
+    const uint32_t g = blockIdx.y * 4 + (t >> 6);
+    if (32 * g >= q_stride || b >= n_blocks) return;
+    uint32_t w[32];
 #pragma unroll
-        for (int i = 0; i < 8; i++) dst[i] = u32x4{w[4 * i], w[4 * i + 1], w[4 * i + 2], w[4 * i + 3]};
+    for (int x = 0; x < 32; x++) {
+        const uint32_t q = 32 * g + x;
+        w[x] = q < n_queries ? masks32[static_cast<uint64_t>(q) * stride32 + b] : 0u;
     }
-    __syncthreads();
-    const uint32_t pieces = 8 * groups;  // 16-byte pieces per block row
-    for (uint32_t e = t; e < kMtbBlocks * pieces; e += 256) {
-        const uint32_t bl = e / pieces, pc = e % pieces;
-        if (b0 + bl >= n_blocks) break;
-        const u32x4 v = *reinterpret_cast<const u32x4*>(stage + bl * kMtbOutPitch + 4 * pc);
-        *reinterpret_cast<u32x4*>(out + (b0 + bl) * q_stride + q0 + 4 * pc) = v;
-    }
+    Transpose32(w);
+    u32x4* dst = reinterpret_cast<u32x4*>(out + b * q_stride + 32 * g);
+#pragma unroll
+    for (int i = 0; i < 8; i++) dst[i] = u32x4{w[4 * i], w[4 * i + 1], w[4 * i + 2], w[4 * i + 3]};
 }
 
 }  // namespace
@@ -201,14 +170,15 @@ extern "C" int frk_masks_to_blocks(const uint64_t* d_masks, uint64_t mask_stride
     }
     const uint64_t n_blocks = (n_docs + 31) / 32;
     if (n_blocks == 0 || q_stride == 0) return FRK_OK;
-    if (q_stride / 32 / frk::kMtbGroups > 65535u || (n_blocks + frk::kMtbBlocks - 1) / frk::kMtbBlocks > 0x7FFFFFFFu) {
+    if (q_stride / 32 > 4u * 65535u || (n_blocks + 63) / 64 > 0x7FFFFFFFu) {
         frk::SetError("frk_masks_to_blocks: %u queries exceed the grid", q_stride);
         return FRK_ERR_UNSUPPORTED;
     }
-    const uint32_t tiles = (q_stride / 32 + frk::kMtbGroups - 1) / frk::kMtbGroups;
-    dim3 grid(static_cast<uint32_t>((n_blocks + frk::kMtbBlocks - 1) / frk::kMtbBlocks), tiles);
-    hipLaunchKernelGGL(frk::MasksToBlocksKernel, grid, dim3(256), 0, static_cast<hipStream_t>(stream), d_masks,
-                       mask_stride, n_queries, n_blocks, d_block_masks, q_stride);
+    // 64 blocks x 4 query groups per workgroup
+    dim3 grid(static_cast<uint32_t>((n_blocks + 63) / 64), (q_stride / 32 + 3) / 4);
+    hipLaunchKernelGGL(frk::MasksToBlocksKernel, grid, dim3(256), 0, static_cast<hipStream_t>(stream),
+                       reinterpret_cast<const uint32_t*>(d_masks), 2 * mask_stride, n_queries, n_blocks, d_block_masks,
+                       q_stride);
     FRK_HIP_CHECK(hipGetLastError());
     return FRK_OK;
 }

@@ -167,6 +167,15 @@ def test_driver_unfused_order_and_split_extract_match_fused(setup):
             assert np.array_equal(res[q][0], fused[q][0]) and np.array_equal(res[q][1], fused[q][1]), (extra, q)
 
 
+def test_driver_f16x2_store_matches_oracle(setup):
+    """--dtype f16x2: the plain method on the F16X2 split of the FP32 vectors (built at load), fused and
+    in the unfused order, held to the FP32 path's bounds against the oracle."""
+    d = setup["d"]
+    for extra, tag in (([], "_x"), (["--unfused"], "_xu")):
+        _, res = _run(setup, "--index_dir", str(d / "full"), "--batch_size", "32", "--dtype", "f16x2", *extra, tag=tag)
+        _check_oracle(setup, res)
+
+
 def test_driver_paced_latency(setup):
     d = setup["d"]
     out, res = _run(setup, "--index_dir", str(d / "full"), "--batch_size", "8", "--target_qps", "3000",
