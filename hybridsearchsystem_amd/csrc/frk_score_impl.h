@@ -319,20 +319,27 @@ __device__ __forceinline__ void ComputeGroup(FragA&& a, const u32x4 (&buf)[Group
 }
 
 // Survivor queue: a wave appends each (score, doc) that passed its row's floor and filter bit, with the
-// row's index in the workgroup's query group, to a wave-private LDS queue; the queue is flushed in bulk
-// (one counter atomic per entry, all issued before any result is awaited, then the stores), so no global
-// memory access or atomic sits inside the MFMA loop.
+// row's index in the workgroup's query group, to a wave-private LDS queue; the queue is flushed in bulk,
+// so no global memory access or atomic sits inside the MFMA loop. A flush groups its entries by row: LDS
+// atomics rank each entry within its row, one global counter atomic per row present claims that row's
+// span of its query's candidate list, and the entries are stored at span start + rank. A query with tens
+// of thousands of survivors thus costs one counter atomic per flush instead of one per survivor (those
+// atomics all hit one address and serialise there), and its keys land contiguously.
 #ifndef FRK_SCORE_QUEUE_KEYS
 #define FRK_SCORE_QUEUE_KEYS 256  // entries per wave (9 bytes each)
 #endif
 constexpr int kQueueKeys = FRK_SCORE_QUEUE_KEYS;
-constexpr int kQueueBytes = kQueueKeys * 9;  // per wave: kQueueKeys x (score, doc) u32 pairs, then the rows
+constexpr int kQueueRows = 128;  // rows of a query group (32 RB, RB <= 4)
+// per wave: kQueueKeys x (score, doc) u32 pairs, then the rows, then a u32 per row (count, then offset)
+constexpr int kQueueBytes = kQueueKeys * 9 + kQueueRows * 4;
 static_assert(kQueueKeys % 64 == 0 && kQueueKeys >= 64, "the queue flushes whole waves of entries");
+static_assert((kQueueKeys * 9) % 4 == 0, "the row counters are u32-aligned");
 
 struct WaveQueue {
-    uint2* ent;    // [kQueueKeys]: (score bits, doc)
-    uint8_t* row;  // [kQueueKeys]: accumulator row (query index within the workgroup's query group)
-    uint32_t n;    // uniform across the wave
+    uint2* ent;         // [kQueueKeys]: (score bits, doc)
+    uint8_t* row;       // [kQueueKeys]: accumulator row (query index within the workgroup's query group)
+    uint32_t* row_at;   // [kQueueRows]: during a flush, each row's entry count, then its span's start
+    uint32_t n;         // uniform across the wave
 };
 
 __device__ __forceinline__ void WaveSync() {
@@ -346,16 +353,34 @@ __device__ __forceinline__ float InvPow2(float s) { return __uint_as_float(0x7F0
 // q_scale: the per-row query scales whose factor an entry's score still holds (F16X2), else null.
 __device__ __forceinline__ void FlushQueue(const ScoreArgs& A, uint32_t q0, WaveQueue& Qu, const float* q_scale) {
     const int lane = threadIdx.x & 63;
+    constexpr int kRounds = kQueueKeys / 64;
     WaveSync();
-    for (uint32_t i0 = 0; i0 < Qu.n; i0 += 64) {
-        const uint32_t i = i0 + lane;
+    for (int r = lane; r < kQueueRows; r += 64) Qu.row_at[r] = 0;
+    WaveSync();
+    // each entry's rank within its row
+    uint32_t rank[kRounds];
+#pragma unroll
+    for (int k = 0; k < kRounds; k++) {
+        const uint32_t i = 64 * k + lane;
+        rank[k] = i < Qu.n ? atomicAdd(&Qu.row_at[Qu.row[i]], 1u) : 0u;
+    }
+    WaveSync();
+    // one counter atomic per row present: the start of its span in the query's candidate list
+    for (int r = lane; r < kQueueRows; r += 64) {
+        const uint32_t c = Qu.row_at[r];
+        if (c) Qu.row_at[r] = atomicAdd(&A.counts[static_cast<uint64_t>(q0 + r) * A.counts_stride], c);
+    }
+    WaveSync();
+#pragma unroll
+    for (int k = 0; k < kRounds; k++) {
+        const uint32_t i = 64 * k + lane;
         if (i < Qu.n) {
             const uint2 e = Qu.ent[i];
             const uint32_t r = Qu.row[i];
             const uint32_t q = q0 + r;
             float sc = __uint_as_float(e.x);
             if (q_scale) sc *= InvPow2(q_scale[r]);
-            const uint32_t at = atomicAdd(&A.counts[static_cast<uint64_t>(q) * A.counts_stride], 1u);
+            const uint32_t at = Qu.row_at[r] + rank[k];
             if (at < A.cap) A.cand[static_cast<uint64_t>(q) * A.cap + at] = MakeKey(sc, e.y);
         }
     }
@@ -581,6 +606,7 @@ __global__ __launch_bounds__((WgShape<DT, CH, RB>::kThreads), 1) void ScoreKerne
     constexpr int G = GroupShape<DT, CH>::G;
     constexpr int NG = GroupShape<DT, CH>::kGroups;
     constexpr int kRingN = GroupShape<DT, CH>::kRingN;
+    static_assert(32 * RB <= kQueueRows, "a queue row index is a row of the query group");
     extern __shared__ u32x4 smem[];
     u32x4* a_lds = smem;                                               // [RB][CH][64]
     float* q_scale_lds = reinterpret_cast<float*>(smem + RB * CH * 64);  // [32 * RB]
@@ -619,6 +645,7 @@ __global__ __launch_bounds__((WgShape<DT, CH, RB>::kThreads), 1) void ScoreKerne
     WaveQueue Qu;
     Qu.ent = reinterpret_cast<uint2*>(queue_lds + wave * kQueueBytes);
     Qu.row = queue_lds + wave * kQueueBytes + kQueueKeys * sizeof(uint2);
+    Qu.row_at = reinterpret_cast<uint32_t*>(Qu.row + kQueueKeys);
     Qu.n = 0;
     const uint64_t per_wg = (A.block_count + n_ranges - 1) / n_ranges;
     const uint64_t i_lo = static_cast<uint64_t>(range_x) * per_wg;
