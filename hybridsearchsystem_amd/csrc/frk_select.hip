@@ -70,6 +70,25 @@ __device__ inline void FindDigit(const uint32_t* hist, uint32_t rem, uint32_t* d
     }
 }
 
+// f(key) for the keys src[0 .. n), each thread taking every kThreads-th: eight independent loads in flight
+// per thread before any is used (a pass over tens of thousands of keys in global memory is bound by load
+// latency, not bandwidth).
+constexpr int kKeyUnroll = 8;
+template <typename F>
+__device__ __forceinline__ void ForEachKey(const uint64_t* __restrict__ src, uint32_t n, F&& f) {
+    for (uint32_t base = 0; base < n; base += kThreads * kKeyUnroll) {
+        uint64_t x[kKeyUnroll];
+#pragma unroll
+        for (int u = 0; u < kKeyUnroll; u++) {
+            const uint32_t i = base + u * kThreads + threadIdx.x;
+            x[u] = i < n ? src[i] : 0ull;
+        }
+#pragma unroll
+        for (int u = 0; u < kKeyUnroll; u++)
+            if (base + u * kThreads + threadIdx.x < n) f(x[u]);
+    }
+}
+
 // kSorted: the k keys sorted descending; otherwise the same k keys in no particular order (what a
 // caller that rescores them needs), which skips the final sort.
 template <bool kSorted>
@@ -131,11 +150,10 @@ __global__ __launch_bounds__(kThreads) void SelectKernel(const uint64_t* __restr
     __syncthreads();
     {
         uint64_t lo = ~0ull, hi = 0ull;
-        for (uint32_t i = threadIdx.x; i < n; i += kThreads) {
-            const uint64_t key = c[i];
+        ForEachKey(c, n, [&](uint64_t key) {
             lo = key < lo ? key : lo;
             hi = key > hi ? key : hi;
-        }
+        });
         for (int off = 32; off > 0; off >>= 1) {
             const uint64_t l2 = __shfl_xor(lo, off, 64), h2 = __shfl_xor(hi, off, 64);
             lo = l2 < lo ? l2 : lo;
@@ -163,13 +181,14 @@ __global__ __launch_bounds__(kThreads) void SelectKernel(const uint64_t* __restr
         for (int b = threadIdx.x; b < 256 * kWaves; b += kThreads) whist[b] = 0;
         __syncthreads();
         if (threadIdx.x == 0) s_digit = s_above = 0;
-        auto pass = [&](const uint64_t* src, uint32_t cnt) __attribute__((always_inline)) {
-            for (uint32_t i = threadIdx.x; i < cnt; i += kThreads) {
-                const uint64_t key = src[i];
-                if ((key & pmask) == prefix) atomicAdd(&whist[wave * 256 + ((key >> shift) & 255u)], 1u);
-            }
+        auto count = [&](uint64_t key) {
+            if ((key & pmask) == prefix) atomicAdd(&whist[wave * 256 + ((key >> shift) & 255u)], 1u);
         };
-        if (in_lds) pass(keys, stored); else pass(c, n);
+        if (in_lds) {
+            for (uint32_t i = threadIdx.x; i < stored; i += kThreads) count(keys[i]);
+        } else {
+            ForEachKey(c, n, count);
+        }
         __syncthreads();
         for (int b = threadIdx.x; b < 256; b += kThreads) {
             uint32_t t = 0;
@@ -189,10 +208,9 @@ __global__ __launch_bounds__(kThreads) void SelectKernel(const uint64_t* __restr
             // compact the keys still in the running into LDS
             if (threadIdx.x == 0) s_fill = 0;
             __syncthreads();
-            for (uint32_t i = threadIdx.x; i < n; i += kThreads) {
-                const uint64_t key = c[i];
+            ForEachKey(c, n, [&](uint64_t key) {
                 if ((key & pmask) == prefix) keys[atomicAdd(&s_fill, 1u)] = key;
-            }
+            });
             __syncthreads();
             in_lds = true;
             stored = live;
@@ -203,8 +221,7 @@ __global__ __launch_bounds__(kThreads) void SelectKernel(const uint64_t* __restr
     const uint64_t T = prefix;  // the k-th largest key
     // the k keys >= T (keys are unique, so exactly k): those of bins above the chosen digits were
     // never compacted, so read them all
-    for (uint32_t i = threadIdx.x; i < n; i += kThreads) {
-        const uint64_t key = c[i];
+    ForEachKey(c, n, [&](uint64_t key) {
         if (key >= T) {
             const uint32_t at = atomicAdd(&s_fill, 1u);
             if (kSorted) {
@@ -213,7 +230,7 @@ __global__ __launch_bounds__(kThreads) void SelectKernel(const uint64_t* __restr
                 o[at] = key;
             }
         }
-    }
+    });
     if constexpr (kSorted) {
         __syncthreads();
         uint32_t p2 = 1;
