@@ -816,9 +816,26 @@ int Launch(const ScoreArgs& A, hipStream_t stream, uint32_t grid_x) {
 // every group re-reads the doc range (from the XCD's L2 after the first). Measured on 10M x 768 BF16
 // (with the staged query fragments): 192 queries 4.02 -> 3.53 ms, 128 queries (still two groups of 64)
 // 2.91 -> 2.75 ms from the staging alone.
+}  // namespace frk
+
+#include "frk_score_staged.h"
+
+namespace frk {
+
 template <int DT, int kMode>
 int Dispatch(const ScoreArgs& A, uint32_t ch, hipStream_t stream, uint32_t grid_x) {
     const uint32_t q = A.n_queries;
+    // wide docs and large batches: the staged scorer (one read of the doc range for up to 256 queries)
+#define FRK_STAGED(CHV)                                                        \
+    if (ch == CHV) {                                                           \
+        bool done = false;                                                     \
+        const int rc = TryStaged<DT, CHV, kMode>(A, stream, grid_x, &done);    \
+        if (done) return rc;                                                   \
+    }
+    FRK_STAGED(24)
+    FRK_STAGED(32)
+    FRK_STAGED(48)
+#undef FRK_STAGED
     int rb = q > 64 ? 4 : (q > 32 ? 2 : 1);
     if (rb > FRK_SCORE_MAX_RB) rb = FRK_SCORE_MAX_RB;
     // FP8 probe pass: two query groups of two row blocks (sharing an XCD's L2) measured faster

@@ -379,6 +379,66 @@ def test_low_precision_scores_match_dequantized_values(frk, dtype, dim, nq):
         assert np.all(np.abs(sc - exact) <= tol * scale + 1e-30), (q, np.max(np.abs(sc - exact) / scale))
 
 
+@pytest.mark.parametrize("dtype,dim,nq", [(1, 768, 256), (1, 768, 100), (1, 384, 300), (1, 512, 130), (2, 768, 200),
+                                          (2, 1024, 70)])
+def test_staged_scorer_equals_small_batches(frk, dtype, dim, nq):
+    """The scorer for wide docs and large batches (doc tiles staged in LDS, query fragments in registers;
+    BF16 384-768 dims from 97 queries on, FP8 768-1024 from 65) against the same queries scored in batches
+    of at most 64, which the LDS-query-tile scorer serves: per query the surviving (score, doc) keys
+    under a floor, and the per-block maxima of the probe pass, are bit-identical (same conversions, same
+    MFMA chain). The shapes cover one and two query groups (300 = 256 + 44), waves past the batch in
+    both workgroup sizes (100, 130, 200), a ragged last block, and floors at which part of each block
+    survives, so the survivor queue fills and flushes."""
+    import ctypes
+    from hybridsearchsystem_amd import check, lib
+    print("staged scorer dtype %d dim %d nq %d" % (dtype, dim, nq), flush=True)
+    rng = np.random.default_rng(900 + dtype + dim + nq)
+    n = 6000 + 13
+    docs = rng.standard_normal((n, dim)).astype(np.float32)
+    qs = rng.standard_normal((nq, dim)).astype(np.float32)
+    bits = rng.random((nq, n)) < rng.uniform(0.05, 0.95, (nq, 1))
+    tiles, scales, dp = frk.tile_docs(docs, dtype)
+    assert dp == dim
+    d_t, d_s = dev(tiles), dev(scales) if scales is not None else None
+    store = frk.doc_store(d_t, d_s, dtype, dp, n)
+    d_q, d_m = dev(qs), dev(frk.block_masks_host(bits))
+    stride = d_m.shape[1]
+    # floors near each query's 70th-99th percentile of dot products: a few hundred survivors per query
+    s64 = docs.astype(np.float64) @ qs.astype(np.float64).T
+    floor = np.array([np.quantile(s64[:, q], rng.uniform(0.7, 0.99)) for q in range(nq)], np.float32)
+    d_f = dev(floor)
+    nb = (n + 31) // 32
+    runs = []
+    for batches in ([(0, nq)], [(q0, min(64, nq - q0)) for q0 in range(0, nq, 64)]):
+        d_cand = torch.zeros((nq, n), dtype=torch.int64, device="cuda")
+        d_cnt = torch.zeros(nq, dtype=torch.int32, device="cuda")
+        d_bm = torch.zeros((nq, nb), dtype=torch.float32, device="cuda")
+        for q0, m in batches:
+            check(lib().frk_score_select(ctypes.byref(store), d_q[q0].data_ptr(), m, d_m[:, q0:].data_ptr(), stride,
+                                         d_f[q0].data_ptr(), 0, 1, nb, d_cand[q0].data_ptr(), n, d_cnt[q0].data_ptr(),
+                                         1, None), "score_select")
+            check(lib().frk_score_blockmax(ctypes.byref(store), d_q[q0].data_ptr(), m, d_m[:, q0:].data_ptr(), stride,
+                                           0, 1, nb, d_bm[q0].data_ptr(), nb, None), "blockmax")
+        torch.cuda.synchronize()
+        runs.append((d_cnt.cpu().numpy(), d_cand.cpu().numpy().view(np.uint64), d_bm.cpu().numpy()))
+    (c1, k1, b1), (c2, k2, b2) = runs
+    assert np.array_equal(c1, c2)
+    assert c1.min() > 0 and c1.max() < n  # floors leave part of the docs
+    for q in range(nq):
+        assert np.array_equal(np.sort(k1[q, :c1[q]]), np.sort(k2[q, :c2[q]])), q
+    assert np.array_equal(b1.view(np.uint32), b2.view(np.uint32))
+    # and the staged launch against the FP64 dot products of the stored values
+    dd, dq = _dequantized(docs, dtype), _dequantized(qs, dtype)
+    for q in range(0, nq, 7):
+        sc, ids = frk.decode_keys(k1[q, :c1[q]])
+        want = np.flatnonzero(bits[q] & (dd @ dq[q] >= floor[q]))
+        exact = dd[ids] @ dq[q]
+        assert np.all(np.abs(sc - exact) <= 4e-5 * np.abs(dd[ids] * dq[q]).sum(axis=1))
+        # docs within rounding of the floor may fall either side
+        near = np.abs(dd @ dq[q] - floor[q]) <= 4e-5 * np.abs(dd * dq[q]).sum(axis=1)
+        assert near[np.setxor1d(ids, want)].all(), q
+
+
 def test_rescore_from_tiles_and_rows(frk):
     """FP32 rescoring of candidate keys, from the tiled store and from row-major rows: both give each
     candidate its doc and the FP32 dot product within 1e-5 of sum|q_d v_d| (FP64 truth); counts past the
