@@ -25,6 +25,8 @@ from . import check, lib
 from ._capi_sigs import SearchOptions
 
 HBM_PEAK = 8.0e12
+# published dense MFMA peaks of an MI355X per store dtype (FP32, BF16, FP8; F16X2 runs FP16 MFMAs)
+MFMA_PEAK = {0: 157.3e12, 1: 2.5e15, 2: 5.0e15, 3: 2.5e15}
 
 
 class ServerConfig(ctypes.Structure):
@@ -201,11 +203,12 @@ def main():
     ap.add_argument("--docs", type=int, default=20_000_000)
     ap.add_argument("--dim", type=int, default=768)
     ap.add_argument("--tags", type=int, default=4096)
-    ap.add_argument("--rates", default="500,2000,5000,10000,20000", help="offered queries/s")
+    ap.add_argument("--rates", default="500,2000,5000,10000,20000,30000,40000", help="offered queries/s")
     ap.add_argument("--plain-rates", default="500,2000,5000", help="offered queries/s for the plain FP32 path")
     ap.add_argument("--seconds", type=float, default=8.0, help="offered load per rate")
     ap.add_argument("--train", type=int, default=50_000, help="learned-floor training queries")
-    ap.add_argument("--workers", type=int, default=2)
+    ap.add_argument("--workers", type=int, default=4, help="batches in flight (measured at saturation, 20M x 768: "
+                    "BF16 18.0k / 19.5k / 20.1k q/s and FP8 27.8k / 30.5k / 33.0k with 2 / 3 / 4)")
     ap.add_argument("--dtypes", default="bf16,fp8", help="low-precision stores to serve from, each with the "
                     "learned floor and FP32 rescoring")
     ap.add_argument("--plain", action="store_true", help="also run the plain FP32 path at each rate")
@@ -278,6 +281,9 @@ def main():
             n_req = int(min(2000, max(40, rate * args.seconds / 128.5)))
             log("%s: offered %.0f queries/s, %d requests" % (name, rate, n_req))
             res, _, _ = poisson_load(srv, qs, fs, rate, n_req, seed=int(rate))
+            # the main pass's multiply-adds against the store dtype's MFMA peak (the same queries'
+            # doc bytes against HBM are hbm_roofline_frac)
+            res["mfma_frac"] = round(2.0 * res["queries"] * args.docs * args.dim / res["wall_s"] / MFMA_PEAK[kw["dtype"]], 4)
             res.update(pipeline=name, config={"workload": "serving %d x %d, %d tags" % (args.docs, args.dim, args.tags),
                                               "workers": args.workers, "max_batch_queries": 256})
             print(json.dumps(res), flush=True)
