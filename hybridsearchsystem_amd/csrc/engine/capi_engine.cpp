@@ -382,6 +382,7 @@ int frk_server_create(const frk_table* t, const char* vector_field, const char* 
         if (cfg->workers) c.workers = cfg->workers;
         c.hbmBudgetBytes = cfg->hbm_budget_bytes;
         if (cfg->pad_queries) c.padQueries = cfg->pad_queries;
+        c.splitRequests = cfg->split_requests != 0;
     }
     VectorFieldData* vf = nullptr;
     if (!t->table->GetVectorFieldData(c.vectorField, vf) || !vf)

@@ -101,9 +101,16 @@ uint64_t SearchServer::Submit(uint32_t n, const float* queries, uint32_t dim, co
         }
         r->trees[i] = BuildQueryNode(ast.get(), m_cfg.postingField);
     }
+    RequestResult& o = r->result;
+    o.ok = true;  // until a batch holding some of its queries fails
+    o.counts.assign(n, 0);
+    o.ids.assign(n, {});
+    o.scores.assign(n, {});
+    o.passing.assign(n, 0);
+    o.started = -1.0;
     std::lock_guard<std::mutex> g(m_mu);
     r->ticket = m_nextTicket++;
-    r->result.submitted = Now();
+    o.submitted = Now();
     m_tickets[r->ticket] = r;
     m_pending.push_back(r);
     m_cv.notify_all();
@@ -148,19 +155,27 @@ void SearchServer::SchedulerLoop() {
             return (m_stop && m_pending.empty()) || (!m_pending.empty() && free >= 0);
         });
         if (m_stop && m_pending.empty()) return;
-        // the oldest requests that fit one batch's query slots ...
-        size_t take = 0;
+        // the oldest pending queries that fit one batch's query slots: whole requests (one larger
+        // than a batch runs alone), or with splitRequests the remainder of the oldest request first
+        // and the last request cut at the batch's end ...
+        std::vector<Piece> pieces;
         uint32_t q = 0;
         for (const auto& r : m_pending) {
-            if (take > 0 && q + r->n > m_cfg.maxBatchQueries) break;
-            q += r->n;
-            take++;
+            const uint32_t left = r->n - r->scheduled;
+            uint32_t take = left;
+            if (q + left > m_cfg.maxBatchQueries) {
+                if (!m_cfg.splitRequests && !pieces.empty()) break;
+                if (m_cfg.splitRequests) take = m_cfg.maxBatchQueries - q;
+            }
+            pieces.push_back(Piece{r, r->scheduled, take});
+            q += take;
+            if (q >= m_cfg.maxBatchQueries) break;
         }
         // ... and the HBM left by the batches in flight (one batch always runs on an idle device)
         uint64_t bytes = BatchBytes(q);
-        while (m_cfg.hbmBudgetBytes && take > 1 && m_bytesInFlight + bytes > m_cfg.hbmBudgetBytes) {
-            q -= m_pending[take - 1]->n;
-            take--;
+        while (m_cfg.hbmBudgetBytes && pieces.size() > 1 && m_bytesInFlight + bytes > m_cfg.hbmBudgetBytes) {
+            q -= pieces.back().count;
+            pieces.pop_back();
             bytes = BatchBytes(q);
         }
         if (m_cfg.hbmBudgetBytes && m_bytesInFlight + bytes > m_cfg.hbmBudgetBytes && m_running > 0) {
@@ -168,10 +183,13 @@ void SearchServer::SchedulerLoop() {
             continue;
         }
         auto b = std::make_unique<Batch>();
-        for (size_t i = 0; i < take; i++) {
-            b->reqs.push_back(m_pending.front());
-            m_pending.pop_front();
+        for (const Piece& p : pieces) {
+            p.req->scheduled += p.count;
+            p.req->inFlight++;
+            // pieces come from the front in order; a request leaves the queue once all of it is taken
+            if (p.req->scheduled == p.req->n) m_pending.pop_front();
         }
+        b->pieces = std::move(pieces);
         b->queries = q;
         b->bytes = bytes;
         m_bytesInFlight += bytes;
@@ -193,7 +211,13 @@ void SearchServer::WorkerLoop(uint32_t w) {
         lk.lock();
         m_bytesInFlight -= b->bytes;
         m_running--;
-        for (auto& r : b->reqs) r->done = true;
+        for (const Piece& p : b->pieces) {
+            Request& r = *p.req;
+            if (--r.inFlight == 0 && r.scheduled == r.n) {
+                r.done = true;
+                m_stats.requests++;
+            }
+        }
         m_slot[w].reset();
         m_doneCv.notify_all();
         m_cv.notify_all();
@@ -205,10 +229,11 @@ void SearchServer::RunBatch(Batch& b) {
     FullRecallSearcher s(m_deviceId, m_table);
     s.SetOptions(m_cfg.options);
     uint32_t dim = 0;
-    for (auto& r : b.reqs) {
-        dim = r->dim;
-        for (uint32_t i = 0; i < r->n; i++)
-            s.AddQuery(r->trees[i].get(), &r->queries[static_cast<size_t>(i) * r->dim], r->dim, r->ks[i]);
+    for (const Piece& p : b.pieces) {
+        const Request& r = *p.req;
+        dim = r.dim;
+        for (uint32_t i = p.first; i < p.first + p.count; i++)
+            s.AddQuery(r.trees[i].get(), &r.queries[static_cast<size_t>(i) * r.dim], r.dim, r.ks[i]);
     }
     // padding: zero vectors with k = 1 over an inert term (no doc passes, no survivors, no floor)
     const uint32_t pad = (m_cfg.padQueries - b.queries % m_cfg.padQueries) % m_cfg.padQueries;
@@ -220,18 +245,12 @@ void SearchServer::RunBatch(Batch& b) {
     const double finished = Now();
     const BatchStats st = s.LastStats();
     const std::vector<uint32_t>& passing = s.LastPassing();
+    // this batch's queries of each request (entries no other batch writes; the vectors were sized at
+    // submission)
     size_t at = 0;
-    for (auto& r : b.reqs) {
-        RequestResult& o = r->result;
-        o.ok = ok;
-        o.started = started;
-        o.finished = finished;
-        o.batchQueries = b.queries;
-        o.counts.assign(r->n, 0);
-        o.ids.assign(r->n, {});
-        o.scores.assign(r->n, {});
-        o.passing.assign(r->n, 0);
-        for (uint32_t i = 0; i < r->n && ok; i++, at++) {
+    for (const Piece& p : b.pieces) {
+        RequestResult& o = p.req->result;
+        for (uint32_t i = p.first; i < p.first + p.count && ok; i++, at++) {
             o.ids[i] = res[at]->docIds;
             o.scores[i] = res[at]->scores;
             o.counts[i] = static_cast<uint32_t>(res[at]->docIds.size());
@@ -244,7 +263,15 @@ void SearchServer::RunBatch(Batch& b) {
         docBytes = (store->n_docs + 31) / 32 * 32 * store->dim * elem * (st.reruns ? 2 : 1);
     }
     std::lock_guard<std::mutex> g(m_mu);
-    m_stats.requests += b.reqs.size();
+    for (const Piece& p : b.pieces) {
+        RequestResult& o = p.req->result;
+        o.ok = o.ok && ok;
+        o.started = o.started < 0 ? started : std::min(o.started, started);
+        if (finished >= o.finished) {
+            o.finished = finished;
+            o.batchQueries = b.queries;
+        }
+    }
     m_stats.queries += b.queries;
     m_stats.batches++;
     m_stats.graphReplays += st.graphReplayed ? 1 : 0;

@@ -5,9 +5,10 @@
 // queries (each with its own filter and k) at any time; a scheduler thread packs pending requests,
 // oldest first, into batches of at most maxBatchQueries queries and hands each to the first free
 // worker as soon as one is free and the batch's device working set fits the HBM budget left by the
-// batches in flight. Each worker owns a searcher (its own workspace and stream), so several batches
-// run on the device at once; the index stays resident and is shared read-only. Results come back per
-// request with submit, start and finish times.
+// batches in flight (optionally splitting a request over consecutive batches to fill each one).
+// Each worker owns a searcher (its own workspace and stream), so several batches run on the device
+// at once; the index stays resident and is shared read-only. Results come back per request with
+// submit, start and finish times.
 #pragma once
 
 #include <chrono>
@@ -35,6 +36,8 @@ struct ServerConfig {
     uint64_t hbmBudgetBytes = 0;      // device working memory for all batches in flight; 0 = unlimited
     uint32_t padQueries = 64;         // batches are padded with inert queries to a multiple of this,
                                       // so replayable graph shapes repeat; 1 = no padding
+    bool splitRequests = false;       // a request's queries may span consecutive batches, so that under
+                                      // load every batch is filled to maxBatchQueries
 };
 
 struct RequestResult {
@@ -44,7 +47,8 @@ struct RequestResult {
     std::vector<std::vector<float>> scores;
     std::vector<uint32_t> passing;               // per query: docs that passed the filter
     double submitted = 0, started = 0, finished = 0;  // seconds on the server's steady clock
-    uint32_t batchQueries = 0;                   // queries in the device batch that served it
+    uint32_t batchQueries = 0;                   // queries in the device batch that served it (the one
+                                                 // that finished last, for a request split over several)
 };
 
 struct ServerStats {
@@ -79,11 +83,18 @@ class SearchServer {
         std::vector<float> queries;
         std::vector<std::unique_ptr<QueryNode>> trees;
         std::vector<uint32_t> ks;
-        RequestResult result;
+        RequestResult result;    // per-query vectors sized at submission; batches fill their queries
+        uint32_t scheduled = 0;  // queries handed to batches so far
+        uint32_t inFlight = 0;   // batches holding some of its queries that have not finished
         bool done = false;
     };
+    // queries [first, first + count) of one request
+    struct Piece {
+        std::shared_ptr<Request> req;
+        uint32_t first = 0, count = 0;
+    };
     struct Batch {
-        std::vector<std::shared_ptr<Request>> reqs;
+        std::vector<Piece> pieces;
         uint32_t queries = 0;
         uint64_t bytes = 0;
     };

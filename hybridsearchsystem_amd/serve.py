@@ -31,7 +31,8 @@ MFMA_PEAK = {0: 157.3e12, 1: 2.5e15, 2: 5.0e15, 3: 2.5e15}
 
 class ServerConfig(ctypes.Structure):
     _fields_ = [("max_batch_queries", ctypes.c_uint32), ("workers", ctypes.c_uint32),
-                ("hbm_budget_bytes", ctypes.c_uint64), ("pad_queries", ctypes.c_uint32)]
+                ("hbm_budget_bytes", ctypes.c_uint64), ("pad_queries", ctypes.c_uint32),
+                ("split_requests", ctypes.c_uint32)]
 
 
 class ServerStats(ctypes.Structure):
@@ -68,14 +69,14 @@ def _bind(L):
 
 class Server:
     def __init__(self, table, dtype=0, rescore_factor=4, probe=True, floor_model=None, graph=True,
-                 max_batch_queries=256, workers=2, hbm_budget_bytes=0, pad_queries=64):
+                 max_batch_queries=256, workers=2, hbm_budget_bytes=0, pad_queries=64, split_requests=False):
         L = _bind(lib())
         self.table = table
         self._model = floor_model.c_model() if floor_model is not None else None  # must outlive the server
         opts = SearchOptions(dtype, rescore_factor, int(bool(probe)), 0, None, 0, None, int(bool(graph)))
         if self._model is not None:
             opts.floor_model = ctypes.cast(ctypes.byref(self._model), ctypes.c_void_p)
-        cfg = ServerConfig(max_batch_queries, workers, hbm_budget_bytes, pad_queries)
+        cfg = ServerConfig(max_batch_queries, workers, hbm_budget_bytes, pad_queries, int(bool(split_requests)))
         h = ctypes.c_void_p()
         check(L.frk_server_create(table.h, table.vector_field.encode(), table.posting_field.encode(),
                                   ctypes.byref(opts), ctypes.byref(cfg), ctypes.byref(h)), "frk_server_create")
@@ -209,6 +210,8 @@ def main():
     ap.add_argument("--train", type=int, default=50_000, help="learned-floor training queries")
     ap.add_argument("--workers", type=int, default=4, help="batches in flight (measured at saturation, 20M x 768: "
                     "BF16 18.0k / 19.5k / 20.1k q/s and FP8 27.8k / 30.5k / 33.0k with 2 / 3 / 4)")
+    ap.add_argument("--whole-requests", action="store_true", help="never split a request over batches (by "
+                    "default the scheduler fills every batch to 256 queries, cutting the last request)")
     ap.add_argument("--dtypes", default="bf16,fp8", help="low-precision stores to serve from, each with the "
                     "learned floor and FP32 rescoring")
     ap.add_argument("--plain", action="store_true", help="also run the plain FP32 path at each rate")
@@ -275,7 +278,7 @@ def main():
         runs.append(("plain fp32, graph replay", args.plain_rates,
                      dict(dtype=search.FRK_DTYPE_F32, probe=True, graph=True)))
     for name, rates, kw in runs:
-        srv = Server(table, workers=args.workers, **kw)
+        srv = Server(table, workers=args.workers, split_requests=not args.whole_requests, **kw)
         poisson_load(srv, qs, fs, 2000.0, 20, seed=99)  # warm: workspaces grown, graphs captured
         for rate in [float(r) for r in rates.split(",")]:
             n_req = int(min(2000, max(40, rate * args.seconds / 128.5)))
@@ -285,7 +288,8 @@ def main():
             # doc bytes against HBM are hbm_roofline_frac)
             res["mfma_frac"] = round(2.0 * res["queries"] * args.docs * args.dim / res["wall_s"] / MFMA_PEAK[kw["dtype"]], 4)
             res.update(pipeline=name, config={"workload": "serving %d x %d, %d tags" % (args.docs, args.dim, args.tags),
-                                              "workers": args.workers, "max_batch_queries": 256})
+                                              "workers": args.workers, "max_batch_queries": 256,
+                                              "split_requests": not args.whole_requests})
             print(json.dumps(res), flush=True)
         srv.close()
 

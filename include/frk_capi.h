@@ -465,6 +465,8 @@ typedef struct {
     uint64_t hbm_budget_bytes;  /* 0 = unlimited */
     uint32_t pad_queries;       /* pad batches with inert queries to a multiple of this (graph reuse);
                                    0 = 64, 1 = no padding */
+    uint32_t split_requests;    /* 1 = a request's queries may span consecutive batches, so every batch
+                                   is filled to max_batch_queries under load; 0 = whole requests */
 } frk_server_config;
 
 typedef struct {

@@ -112,3 +112,41 @@ def test_server_result_same_alone_and_in_mix(corpus, budget_queries):
     srv.close()
     if budget_queries:
         assert serve.pool_stats()["retain_limit"] == 0
+
+
+def test_server_splits_requests_over_batches(corpus):
+    """With split_requests the scheduler cuts requests at batch boundaries (here 64-query batches and
+    requests of up to 150 queries, so most span two or three batches): every query's ids, scores,
+    counts and passing counts equal those of the same request served whole and alone, no batch
+    exceeds its 64 queries, and every query is served once."""
+    t, qs, fs = corpus
+    from hybridsearchsystem_amd import serve
+    reqs = _requests(np.random.default_rng(71), qs, fs, 20, 150)
+    total = sum(len(r[0]) for r in reqs)
+    print("server split: %d requests, %d queries" % (len(reqs), total), flush=True)
+    srv = serve.Server(t, dtype=1, rescore_factor=4, probe=True, graph=True, workers=2, max_batch_queries=64,
+                       split_requests=True)
+    tickets = [None] * len(reqs)
+
+    def producer():
+        for i, (idx, k) in enumerate(reqs):
+            tickets[i] = srv.submit(qs[idx], [fs[j] for j in idx], k)
+
+    th = threading.Thread(target=producer)
+    th.start()
+    th.join()
+    split = [srv.wait(tk) for tk in tickets]
+    st = srv.stats()
+    srv.close()
+    assert st["queries"] == total and st["requests"] == len(reqs)
+    assert st["batches"] >= (total + 63) // 64
+    assert all(m[5] <= 64 for m in split)
+    assert any(len(r[0]) > 64 for r in reqs)
+    whole = serve.Server(t, dtype=1, rescore_factor=4, probe=True, graph=True, workers=2, max_batch_queries=256)
+    for (idx, k), m in zip(reqs, split):
+        a = whole.wait(whole.submit(qs[idx], [fs[j] for j in idx], k))
+        assert a[5] == len(idx)
+        assert np.array_equal(a[2], m[2]) and np.array_equal(a[0], m[0]) and np.array_equal(a[1], m[1])
+        assert np.array_equal(a[3], m[3])
+        assert m[4][0] <= m[4][1] <= m[4][2]  # submitted <= started <= finished
+    whole.close()
