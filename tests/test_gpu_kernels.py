@@ -408,25 +408,34 @@ def test_staged_scorer_equals_small_batches(frk, dtype, dim, nq):
     floor = np.array([np.quantile(s64[:, q], rng.uniform(0.7, 0.99)) for q in range(nq)], np.float32)
     d_f = dev(floor)
     nb = (n + 31) // 32
+    # the probe's sampling as the searcher runs it: every 3rd block from block 1 (the last sample is the
+    # ragged last block)
+    p_begin, p_step = 1, 3
+    p_cnt = (nb - 1 - p_begin) // p_step + 1
     runs = []
     for batches in ([(0, nq)], [(q0, min(64, nq - q0)) for q0 in range(0, nq, 64)]):
         d_cand = torch.zeros((nq, n), dtype=torch.int64, device="cuda")
         d_cnt = torch.zeros(nq, dtype=torch.int32, device="cuda")
         d_bm = torch.zeros((nq, nb), dtype=torch.float32, device="cuda")
+        d_ps = torch.zeros((nq, p_cnt), dtype=torch.float32, device="cuda")
         for q0, m in batches:
             check(lib().frk_score_select(ctypes.byref(store), d_q[q0].data_ptr(), m, d_m[:, q0:].data_ptr(), stride,
                                          d_f[q0].data_ptr(), 0, 1, nb, d_cand[q0].data_ptr(), n, d_cnt[q0].data_ptr(),
                                          1, None), "score_select")
             check(lib().frk_score_blockmax(ctypes.byref(store), d_q[q0].data_ptr(), m, d_m[:, q0:].data_ptr(), stride,
                                            0, 1, nb, d_bm[q0].data_ptr(), nb, None), "blockmax")
+            check(lib().frk_score_blockmax(ctypes.byref(store), d_q[q0].data_ptr(), m, d_m[:, q0:].data_ptr(), stride,
+                                           p_begin, p_step, p_cnt, d_ps[q0].data_ptr(), p_cnt, None), "sampled blockmax")
         torch.cuda.synchronize()
-        runs.append((d_cnt.cpu().numpy(), d_cand.cpu().numpy().view(np.uint64), d_bm.cpu().numpy()))
-    (c1, k1, b1), (c2, k2, b2) = runs
+        runs.append((d_cnt.cpu().numpy(), d_cand.cpu().numpy().view(np.uint64), d_bm.cpu().numpy(), d_ps.cpu().numpy()))
+    (c1, k1, b1, s1), (c2, k2, b2, s2) = runs
     assert np.array_equal(c1, c2)
     assert c1.min() > 0 and c1.max() < n  # floors leave part of the docs
     for q in range(nq):
         assert np.array_equal(np.sort(k1[q, :c1[q]]), np.sort(k2[q, :c2[q]])), q
     assert np.array_equal(b1.view(np.uint32), b2.view(np.uint32))
+    assert np.array_equal(s1.view(np.uint32), s2.view(np.uint32))
+    assert np.array_equal(s1.view(np.uint32), b1[:, p_begin::p_step].view(np.uint32))
     # and the staged launch against the FP64 dot products of the stored values
     dd, dq = _dequantized(docs, dtype), _dequantized(qs, dtype)
     for q in range(0, nq, 7):
