@@ -574,7 +574,8 @@ def main():
     precision = precision_report(table, runs[:6], head, qs, pf, ks, args) if head is not None else None
     learned_out, model = None, None
     if not args.no_learned:
-        learned_legs = [("bf16", FRK_DTYPE_BF16, 4, 1), ("bf16", FRK_DTYPE_BF16, 4, 2), ("fp8", FRK_DTYPE_FP8E4M3, 4, 1),
+        learned_legs = [("bf16", FRK_DTYPE_BF16, 4, 1), ("bf16", FRK_DTYPE_BF16, 4, 2), ("bf16", FRK_DTYPE_BF16, 4, 3),
+                        ("fp8", FRK_DTYPE_FP8E4M3, 4, 1),
                         ("fp8", FRK_DTYPE_FP8E4M3, 8, 1), ("fp8", FRK_DTYPE_FP8E4M3, 4, 2), ("fp8", FRK_DTYPE_FP8E4M3, 8, 2),
                         ("fp8", FRK_DTYPE_FP8E4M3, 4, 3), ("fp8", FRK_DTYPE_FP8E4M3, 8, 3)]
         learned_out, lruns, model = learned_leg(table, args, streams, res_p, centroids, seed, learned_legs)
@@ -655,12 +656,16 @@ def main():
         # the target's pipeline: BF16 store + learned floor + FP32 rescoring, against both plain paths
         bl = [r for r in runs if r["name"].startswith("bf16 + learned")]
         if bl:
+            # the fastest BF16 leg, re-timed alone over as many batches as the chosen pipeline
             b = max(bl, key=lambda r: r["qps"])
+            bt = timed_leg(table, qs, pf, ks, args, res_p, b["name"].split(", ")[0], b["dtype"], streams=b["streams"],
+                           steps=args.steps, **{k: v for k, v in b["kw"].items() if k not in ("dtype", "graph")})
             out["target_bf16_learned"] = {
-                "pipeline": b["name"],
-                "qps": round(b["qps"], 2), "recall_at_k_vs_plain": round(b["recall"], 5),
-                "vs_plain_fp32_mfma": round(b["qps"] / plain["qps"], 3),
-                "vs_plain_f16x2": round(b["qps"] / plain_x["qps"], 3)}
+                "pipeline": b["name"], "steps": args.steps,
+                "qps": round(bt["qps"], 2), "p99_ms": round(float(np.percentile(bt["lat"], 99)), 3),
+                "recall_at_k_vs_plain": round(bt["recall"], 5),
+                "vs_plain_fp32_mfma": round(bt["qps"] / plain["qps"], 3),
+                "vs_plain_f16x2": round(bt["qps"] / plain_x["qps"], 3)}
     if not args.no_cpu:
         nq = min(args.cpu_queries, args.queries)
         words = synth.doc_tag_words(off, ids, args.docs, N_LOGU_TAGS)  # the headline filters use tags 0..63
