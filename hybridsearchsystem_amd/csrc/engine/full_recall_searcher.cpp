@@ -648,11 +648,17 @@ bool FullRecallSearcher::RunDevice(const VectorFieldData& vf, std::vector<AggrDe
     // keeps ~k' * step survivors per query while the probe reads 1 / step of the store. Automatic step:
     // ~8,192 survivors for the batch's largest k' (a survivor costs the main pass far less than a
     // sampled block costs the probe), at least 4 sampled blocks per key, every 2nd to every 64th block.
+    // Batches below 32 queries (the reruns of overshooting floors among them) aim at proportionally
+    // more survivors per query: their main pass reads the whole store for a few queries anyway, so
+    // the probe's share is what a larger step saves (one query at k' = 4,000 over 20M docs: the probe
+    // reads 1/39 of the store instead of half of it).
     uint32_t probeStep = m_options.probeStep;
     const bool autoStep = probeStep == 0;
-    if (autoStep)
+    if (autoStep) {
+        const uint64_t target = 8192ull * std::max<uint64_t>(1, 32 / std::max<uint32_t>(Q, 1));
         probeStep = static_cast<uint32_t>(std::min<uint64_t>(
-            64, std::max<uint64_t>(2, std::min<uint64_t>(8192 / kSel, nBlocks / (4ull * kSel)))));
+            64, std::max<uint64_t>(2, std::min<uint64_t>(target / kSel, nBlocks / (4ull * kSel)))));
+    }
     // a query whose k*m exceeds the sampled blocks gets a -inf floor (no pruning), which stays exact
     uint32_t kSelMin = kSel;
     for (uint32_t k : m_topKs) kSelMin = std::min(kSelMin, kPrime(k));
