@@ -207,7 +207,9 @@ def main():
     ap.add_argument("--rates", default="500,2000,5000,10000,20000,30000,40000", help="offered queries/s")
     ap.add_argument("--plain-rates", default="500,2000,5000", help="offered queries/s for the plain FP32 path")
     ap.add_argument("--seconds", type=float, default=8.0, help="offered load per rate")
-    ap.add_argument("--train", type=int, default=50_000, help="learned-floor training queries")
+    ap.add_argument("--train", type=int, default=150_000, help="learned-floor training queries (measured at "
+                    "saturation, BF16: 50k -> margin 0.164, 24.7k q/s; 150k -> 0.144, 27.2k q/s; 300k -> 0.126, "
+                    "no further gain)")
     ap.add_argument("--workers", type=int, default=4, help="batches in flight (measured at saturation, 20M x 768: "
                     "BF16 18.0k / 19.5k / 20.1k q/s and FP8 27.8k / 30.5k / 33.0k with 2 / 3 / 4)")
     ap.add_argument("--whole-requests", action="store_true", help="never split a request over batches (by "
