@@ -832,6 +832,8 @@ int Dispatch(const ScoreArgs& A, uint32_t ch, hipStream_t stream, uint32_t grid_
         const int rc = TryStaged<DT, CHV, kMode>(A, stream, grid_x, &done);    \
         if (done) return rc;                                                   \
     }
+    FRK_STAGED(8)
+    FRK_STAGED(16)
     FRK_STAGED(24)
     FRK_STAGED(32)
     FRK_STAGED(48)

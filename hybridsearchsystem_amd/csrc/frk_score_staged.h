@@ -26,25 +26,32 @@
 // SIMDs, two of which run two, so a block costs nearly what it does at 256. A third tile buffer and
 // two row blocks per wave (4 waves, 512 registers each) both measured slower; running the epilogue of
 // one of the two waves on each SIMD at the other end of the iteration made no difference.
+//
+// At 256 and 512 dims ScoreKernel holds 128 queries in one LDS tile, so the doc range is read once
+// either way; the staged loop is still as fast or faster from 65 queries (10M docs, no survivors,
+// ScoreKernel -> this): BF16 256 dims 96 queries 0.904 -> 0.882 ms, 128 0.906 -> 0.895, 192 1.297 ->
+// 1.134, 256 1.363 -> 1.287; BF16 512 dims 96 2.155 -> 1.684, 128 1.729 -> 1.711, 192 and 256 equal;
+// FP8 256 dims 96 0.487 -> 0.473, 128 0.495 -> 0.489, 192 0.727 -> 0.648, 256 0.731 -> 0.741. In the
+// BF16 + learned-floor pipeline at 128 queries (~5,000 survivors per query) the BF16 256-dim pass
+// takes 0.978 -> 0.930 ms.
 #pragma once
 
 namespace frk {
 
-// Shapes the staged scorer is built for: BF16 stores with 24-48 chunks per doc (384-768 dims) and FP8
-// ones with 24-32 (768-1024 dims), whose query fragments (4 CH registers) leave room for the
-// accumulators and the LDS read ring under the 256 registers of two waves per SIMD (FP8 at 48 chunks
-// spills there).
+// Shapes the staged scorer is built for: BF16 stores with 16-48 chunks per doc (256-768 dims) and FP8
+// ones with 8 (256 dims) or 24-32 (768-1024 dims). Their query fragments (4 CH registers) leave room
+// for the accumulators and the LDS read ring under the 256 registers of two waves per SIMD (FP8 at 48
+// chunks spills there). FP8 at 16 chunks (512 dims) measured no faster than ScoreKernel from 96
+// queries up (0.908 -> 0.903 ms at 96, 1.25 -> 1.28 ms at 192, 1.28 -> 1.41 ms at 256).
 template <int DT, int CH>
-constexpr bool kStagedShape = (DT == FRK_DTYPE_BF16 || (DT == FRK_DTYPE_FP8E4M3 && CH <= 32)) && CH >= 24 &&
-                              CH <= 48 && CH % 8 == 0;
+constexpr bool kStagedShape = CH % 8 == 0 && ((DT == FRK_DTYPE_BF16 && CH >= 16 && CH <= 48) ||
+                                              (DT == FRK_DTYPE_FP8E4M3 && (CH == 8 || (CH >= 24 && CH <= 32))));
 
-// smallest batches the staged scorer takes (the measurements above)
-#ifndef FRK_SCORE_STAGED_MIN_QUERIES_BF16
-#define FRK_SCORE_STAGED_MIN_QUERIES_BF16 97
-#endif
-#ifndef FRK_SCORE_STAGED_MIN_QUERIES_FP8
-#define FRK_SCORE_STAGED_MIN_QUERIES_FP8 65
-#endif
+// Smallest batches the staged scorer takes (the measurements above and below): BF16 at 48 chunks from
+// 97 queries, the narrower BF16 shapes and FP8 from 65. At 64 queries ScoreKernel is as fast or faster
+// in every shape (10M docs, no survivors: BF16 256 dims 0.866 vs 0.873 ms, 512 dims 1.662 vs 1.676).
+template <int DT, int CH>
+constexpr uint32_t kStagedMinQueries = DT == FRK_DTYPE_BF16 && CH > 32 ? 97u : 65u;
 
 // This wave's query fragments for row block q_row0 .. q_row0 + 31, converted exactly as StageQueries
 // converts them: lane l holds row q_row0 + l % 32, chunk c's elements of lane half l / 32. FP8: the
@@ -317,8 +324,7 @@ template <int DT, int CH, int kMode>
 int TryStaged(const ScoreArgs& A, hipStream_t stream, uint32_t grid_x, bool* done) {
     *done = false;
     if constexpr (kStagedShape<DT, CH> && kMode != kModeDense) {
-        const uint32_t min_q = DT == FRK_DTYPE_BF16 ? FRK_SCORE_STAGED_MIN_QUERIES_BF16 : FRK_SCORE_STAGED_MIN_QUERIES_FP8;
-        if (A.n_queries >= min_q) {
+        if (A.n_queries >= kStagedMinQueries<DT, CH>) {
             *done = true;
             if (A.n_queries <= 128) return LaunchStaged<DT, CH, 4, kMode>(A, stream, grid_x);
             return LaunchStaged<DT, CH, 8, kMode>(A, stream, grid_x);

@@ -380,15 +380,16 @@ def test_low_precision_scores_match_dequantized_values(frk, dtype, dim, nq):
 
 
 @pytest.mark.parametrize("dtype,dim,nq", [(1, 768, 256), (1, 768, 100), (1, 384, 300), (1, 512, 130), (2, 768, 200),
-                                          (2, 1024, 70)])
+                                          (2, 1024, 70), (1, 256, 128), (1, 256, 80), (1, 512, 96), (2, 256, 150),
+                                          (2, 256, 65)])
 def test_staged_scorer_equals_small_batches(frk, dtype, dim, nq):
     """The scorer for wide docs and large batches (doc tiles staged in LDS, query fragments in registers;
-    BF16 384-768 dims from 97 queries on, FP8 768-1024 from 65) against the same queries scored in batches
-    of at most 64, which the LDS-query-tile scorer serves: per query the surviving (score, doc) keys
-    under a floor, and the per-block maxima of the probe pass, are bit-identical (same conversions, same
-    MFMA chain). The shapes cover one and two query groups (300 = 256 + 44), waves past the batch in
-    both workgroup sizes (100, 130, 200), a ragged last block, and floors at which part of each block
-    survives, so the survivor queue fills and flushes."""
+    BF16 256-512 dims and FP8 256 and 768-1024 dims from 65 queries on, BF16 768 dims from 97) against the
+    same queries scored in batches of at most 64, which the LDS-query-tile scorer serves: per query the
+    surviving (score, doc) keys under a floor, and the per-block maxima of the probe pass, are
+    bit-identical (same conversions, same MFMA chain). The shapes cover one and two query groups (300 =
+    256 + 44), waves past the batch in both workgroup sizes (80, 100, 130, 150, 200), a ragged last
+    block, and floors at which part of each block survives, so the survivor queue fills and flushes."""
     import ctypes
     from hybridsearchsystem_amd import check, lib
     print("staged scorer dtype %d dim %d nq %d" % (dtype, dim, nq), flush=True)
