@@ -34,6 +34,14 @@
 // FP8 256 dims 96 0.487 -> 0.473, 128 0.495 -> 0.489, 192 0.727 -> 0.648, 256 0.731 -> 0.741. In the
 // BF16 + learned-floor pipeline at 128 queries (~5,000 survivors per query) the BF16 256-dim pass
 // takes 0.978 -> 0.930 ms.
+//
+// Tried for the 8 and 16 KiB tiles of 256-dim docs and left out: three or four tile buffers, tiles
+// requested two or three blocks ahead with the wait at the top of each block leaving the newest tile
+// in flight (a bare s_barrier, since __syncthreads' workgroup fence waits for every load): FP8 at 128
+// queries 0.497 -> 0.509 / 0.519 ms with no survivors and 0.519 -> 0.534 / 0.565 ms with ~3,100 per
+// query, BF16 0.901 -> 0.944 / 0.933 ms. Fewer workgroups fit a CU, and the compiler makes every LDS
+// write of the survivor queue wait for all of the wave's loads (vmcnt(0): it cannot tell the queue
+// from the tile buffers the global->LDS loads are filling), which then includes the deeper prefetch.
 #pragma once
 
 namespace frk {
