@@ -508,7 +508,7 @@ def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=200, help="batches the chosen pipeline is timed over")
-    ap.add_argument("--select-steps", type=int, default=20, help="batches each candidate pipeline is timed over")
+    ap.add_argument("--select-steps", type=int, default=64, help="batches each candidate pipeline is timed over")
     ap.add_argument("--warmup", type=int, default=5)
     ap.add_argument("--docs", type=int, default=10_000_000)
     ap.add_argument("--dim", type=int, default=256)
