@@ -31,7 +31,8 @@
 // either way; the staged loop is still as fast or faster from 65 queries (10M docs, no survivors,
 // ScoreKernel -> this): BF16 256 dims 96 queries 0.904 -> 0.882 ms, 128 0.906 -> 0.895, 192 1.297 ->
 // 1.134, 256 1.363 -> 1.287; BF16 512 dims 96 2.155 -> 1.684, 128 1.729 -> 1.711, 192 and 256 equal;
-// FP8 256 dims 96 0.487 -> 0.473, 128 0.495 -> 0.489, 192 0.727 -> 0.648, 256 0.731 -> 0.741. In the
+// FP8 256 dims 96 0.487 -> 0.473, 128 0.495 -> 0.489, 192 0.727 -> 0.648, 256 0.731 -> 0.741; BF16 384
+// dims 72 1.456 -> 1.352 ms, 80 1.284 -> 1.267, 96 1.305 -> 1.264 (so 65 there too). In the
 // BF16 + learned-floor pipeline at 128 queries (~5,000 survivors per query) the BF16 256-dim pass
 // takes 0.978 -> 0.930 ms.
 //
