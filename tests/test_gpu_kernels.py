@@ -380,8 +380,8 @@ def test_low_precision_scores_match_dequantized_values(frk, dtype, dim, nq):
 
 
 @pytest.mark.parametrize("dtype,dim,nq", [(1, 768, 256), (1, 768, 100), (1, 384, 300), (1, 512, 130), (2, 768, 200),
-                                          (2, 1024, 70), (1, 256, 128), (1, 256, 80), (1, 512, 96), (2, 256, 150),
-                                          (2, 256, 65)])
+                                          (2, 1024, 70), (1, 256, 128), (1, 256, 80), (1, 256, 300), (1, 512, 96),
+                                          (2, 256, 150), (2, 256, 65)])
 def test_staged_scorer_equals_small_batches(frk, dtype, dim, nq):
     """The scorer for wide docs and large batches (doc tiles staged in LDS, query fragments in registers;
     BF16 256-512 dims and FP8 256 and 768-1024 dims from 65 queries on, BF16 768 dims from 97) against the
